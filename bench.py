@@ -1,0 +1,260 @@
+"""Benchmark of the fftset transform / fast-convolution path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+Prints one JSON line. A "step" is one pass of the hot path over one batch of
+synthetic input already resident in HBM. Throughput (`value`) is algorithmic
+bytes per second: forward / inverse / get_kernel move 16 L bytes per
+transform (2L floats in, 2L out), convolution and its adjoints 24 L (signal,
+kernel spectrum, output); twiddle reads are excluded (shared, cache
+resident). Kernel durations come from HIP events recorded on the stream the
+kernels run on (torch's current stream).
+
+Workloads (complex_bins = L):
+  ragged   20000 transforms, L drawn uniformly from {48, 120, 360, 768, 1440,
+           4608, 15360, 61440}, half COMPLEX forward, half FREQOFFSETREAL
+           convolution, in one launch (the headline workload)
+  c2       COMPLEX L=3840 x 65536, forward + inverse
+  c3       FREQOFFSETREAL conv L=4096 x 16384 channels
+  c5       FREQOFFSETREAL conv L=2048 x 4096 + both adjoints
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "ragged-batch FFT/conv GB/s & %HBM peak vs per-size launches at ≤2e-6 rel RMS"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=["ragged", "c2", "c3", "c5"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
+    return ap.parse_args()
+
+
+class Timer:
+    """HIP events on torch's current stream around each launch."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.samples = {}
+
+    def run(self, name, fn):
+        t = self.torch
+        a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        self.samples.setdefault(name, []).append((a, b))
+
+    def mean_ms(self, name):
+        self.torch.cuda.synchronize()
+        return statistics.fmean(a.elapsed_time(b) for a, b in self.samples[name])
+
+    def reset(self):
+        self.samples = {}
+
+
+def workload_c2(torch, fs, rng_np, args):
+    import fftset_amd
+    L, B = 3840, 65536
+    f = fs.create_fft(fftset_amd.COMPLEX, L)
+    x = torch.randn(B, 2 * L, device="cuda", dtype=torch.float32,
+                    generator=torch.Generator(device="cuda").manual_seed(2))
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    kernels = {
+        "forward": lambda: f.execute(fftset_amd.OP_FORWARD, B, y, x),
+        "inverse": lambda: f.execute(fftset_amd.OP_INVERSE, B, z, y),
+    }
+    bytes_per = {"forward": 16 * L * B, "inverse": 16 * L * B}
+    cfg = {"workload": "uniform COMPLEX forward+inverse", "complex_bins": L, "batch": B,
+           "radices": f.radices()}
+
+    def check():
+        from oracle import Oracle, fp64
+        from oracle.oracle import MOD_COMPLEX
+        o = Oracle().plan(MOD_COMPLEX, L)
+        idx = [0, 1, B // 2, B - 1]
+        xs = x[idx].cpu().numpy()
+        e1 = fp64.rel_rms(y[idx].cpu().numpy(), o.forward(xs))
+        e2 = fp64.rel_rms(z[idx].cpu().numpy(), o.inverse(o.forward(xs)))
+        return float(max(e1.max(), e2.max()))
+
+    def cpu_sample(seconds):
+        return cpu_baseline_uniform(x, L, "cx", ["forward", "inverse"], seconds)
+
+    return kernels, bytes_per, cfg, check, cpu_sample
+
+
+def workload_c3(torch, fs, rng_np, args):
+    import fftset_amd
+    L, B = 4096, 16384
+    f = fs.create_fft(fftset_amd.FREQ_OFFSET_REAL, L)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(B, 2 * L, device="cuda", generator=g) * 2 - 1
+    t = torch.arange(2 * L, device="cuda", dtype=torch.float32)
+    h = torch.randn(B, 2 * L, device="cuda", generator=g) * torch.exp(-t / 512.0)
+    h[:, 4096:] = 0  # 4096-tap kernels in an 8192-sample block
+    K = torch.empty_like(h)
+    f.execute(fftset_amd.OP_GET_KERNEL, B, K, h)
+    y = torch.empty_like(x)
+    kernels = {"conv": lambda: f.execute(fftset_amd.OP_CONV, B, y, x, kern=K)}
+    bytes_per = {"conv": 24 * L * B}
+    cfg = {"workload": "multichannel FREQOFFSETREAL convolution (kernel spectra precomputed)",
+           "complex_bins": L, "channels": B, "kernel_taps": 4096, "radices": f.radices()}
+
+    def check():
+        from oracle import Oracle, fp64
+        from oracle.oracle import MOD_FREQ_OFFSET_REAL
+        o = Oracle().plan(MOD_FREQ_OFFSET_REAL, L)
+        idx = [0, B - 1]
+        xs, hs = x[idx].cpu().numpy(), h[idx].cpu().numpy()
+        return float(fp64.rel_rms(y[idx].cpu().numpy(), o.conv(xs, o.get_kernel(hs))).max())
+
+    def cpu_sample(seconds):
+        return cpu_baseline_uniform(x, L, "fo", ["conv"], seconds, kern_dev=K, h_dev=h)
+
+    return kernels, bytes_per, cfg, check, cpu_sample
+
+
+def workload_c5(torch, fs, rng_np, args):
+    import fftset_amd
+    L, B = 2048, 4096
+    f = fs.create_fft(fftset_amd.FREQ_OFFSET_REAL, L)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(B, 2 * L, device="cuda", generator=g)
+    t = torch.arange(2 * L, device="cuda", dtype=torch.float32)
+    h = torch.randn(B, 2 * L, device="cuda", generator=g) * torch.exp(-t / 256.0)
+    w = torch.randn(B, 2 * L, device="cuda", generator=g)
+    K = torch.empty_like(h)
+    y, gx, gh, work = (torch.empty_like(x) for _ in range(4))
+    kernels = {
+        "get_kernel": lambda: f.execute(fftset_amd.OP_GET_KERNEL, B, K, h),
+        "conv": lambda: f.execute(fftset_amd.OP_CONV, B, y, x, kern=K),
+        "grad_input": lambda: f.execute(fftset_amd.OP_CONV_GRAD_INPUT, B, gx, w, kern=K),
+        "grad_kernel": lambda: f.execute(fftset_amd.OP_CONV_GRAD_KERNEL, B, gh, x, kern=w, work=work),
+    }
+    bytes_per = {"get_kernel": 16 * L * B, "conv": 24 * L * B, "grad_input": 24 * L * B,
+                 "grad_kernel": 24 * L * B}
+    cfg = {"workload": "differentiable FREQOFFSETREAL convolution (forward + both adjoints)",
+           "complex_bins": L, "channels": B, "radices": f.radices()}
+
+    def check():
+        from oracle import fp64
+        idx = [0, B - 1]
+        xs, hs, ws = (a[idx].cpu().numpy() for a in (x, h, w))
+        gx_ref, gh_ref = fp64.fo_conv_grads64(ws, xs, hs)
+        e = max(fp64.rel_rms(gx[idx].cpu().numpy(), gx_ref).max(), fp64.rel_rms(gh[idx].cpu().numpy(), gh_ref).max(),
+                fp64.rel_rms(y[idx].cpu().numpy(), fp64.fo_conv64(xs, hs)).max())
+        return float(e)
+
+    return kernels, bytes_per, cfg, check, None
+
+
+def cpu_baseline_uniform(x_dev, L, mod, ops, seconds, kern_dev=None, h_dev=None):
+    """Oracle (FP32 C restatement of the reference) on the host cores, one
+    transform per thread, over a bounded sample of the same inputs."""
+    import numpy as np
+    from oracle import Oracle
+    from oracle.oracle import MOD_COMPLEX, MOD_FREQ_OFFSET_REAL, OP_CONV, OP_FORWARD, OP_INVERSE
+    threads = min(16, os.cpu_count() or 1)
+    o = Oracle().plan(MOD_COMPLEX if mod == "cx" else MOD_FREQ_OFFSET_REAL, L)
+    n = 64
+    xs = x_dev[:n].cpu().numpy()
+    kern = None
+    if "conv" in ops:
+        kern = o.get_kernel(h_dev[:n].cpu().numpy())
+    opmap = {"forward": OP_FORWARD, "inverse": OP_INVERSE, "conv": OP_CONV}
+    done, t0 = 0, time.perf_counter()
+    bytes_done = 0
+    while time.perf_counter() - t0 < seconds:
+        for op in ops:
+            o.run_batch(opmap[op], xs, kern=kern if op == "conv" else None, nthreads=threads)
+            bytes_done += (24 if op == "conv" else 16) * L * n
+        done += n
+    dt = time.perf_counter() - t0
+    return {"value": round(bytes_done / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{done} transforms x {'+'.join(ops)} at L={L}, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X and this path has no "
+              "multi-GPU component (transforms are independent; see DESIGN.md for how it would shard).",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import fftset_amd
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    fs = fftset_amd.FFTSet()
+    rng = np.random.default_rng(0)
+    build = {"c2": workload_c2, "c3": workload_c3, "c5": workload_c5}[args.workload]
+    kernels, bytes_per, cfg, check, cpu_sample = build(torch, fs, rng, args)
+    timer = Timer(torch)
+
+    def step():
+        for name, fn in kernels.items():
+            timer.run(name, fn)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer.reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = {k: timer.mean_ms(k) for k in kernels}
+    step_ms = sum(ms.values())
+    total_bytes = sum(bytes_per.values())
+    value = total_bytes / (step_ms * 1e-3) / 1e9
+    dom = max(ms, key=ms.get)
+    achieved = bytes_per[dom] / (ms[dom] * 1e-3) / 1e9
+    err = None if args.no_check else check()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4),
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": cfg,
+        "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "max_rel_rms_vs_oracle": err,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu and cpu_sample is not None:
+        out["cpu_baseline"] = cpu_sample(args.cpu_seconds)
+    fs.destroy()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,239 @@
+"""fftset_amd -- MI355X (gfx950) implementation of fftset's transform and
+fast-convolution path behind fftset's C API.
+
+The product is ``libfftset_amd.so`` (HIP kernels + the C ABI declared in
+``include/fftset.h``). This module is a thin ctypes mirror of that API for
+Python callers and tests; buffers are device pointers, passed either as
+integers or as CUDA/HIP torch tensors (float32, contiguous).
+
+Names follow the reference interface (fftset.h): ``FFTSet.create_fft``,
+``FFT.forward``, ``FFT.inverse``, ``FFT.conv_get_kernel``, ``FFT.conv``,
+``recommend_conv_length``, the ``FREQ_OFFSET_REAL`` / ``COMPLEX``
+modulations, plus the extensions (``execute`` for uniform batches, the
+convolution adjoints).
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfftset_amd.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "fftset.h")
+
+OK = 0
+E_INVALID = -1
+E_NOMEM = -2
+E_DEVICE = -3
+E_UNSUPPORTED = -4
+
+OP_FORWARD = 0
+OP_INVERSE = 1
+OP_GET_KERNEL = 2
+OP_CONV = 3
+OP_CONV_GRAD_INPUT = 4
+OP_CONV_GRAD_KERNEL = 5
+
+_lib = None
+
+
+class FFTSetError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"fftset error {code} {what}".strip())
+        self.code = code
+
+
+def lib_available():
+    return os.path.exists(LIB_PATH)
+
+
+def load_library():
+    """Loads libfftset_amd.so; raises when it has not been built (there is
+    no fallback implementation)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m fftset_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, fp, u, sz = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_size_t
+    lib.fftset_init.argtypes = [vp]
+    lib.fftset_init.restype = ctypes.c_int
+    lib.fftset_destroy.argtypes = [vp]
+    lib.fftset_destroy.restype = None
+    lib.fftset_create_fft.argtypes = [vp, vp, u]
+    lib.fftset_create_fft.restype = vp
+    for name in ("fftset_fft_forward", "fftset_fft_inverse"):
+        f = getattr(lib, name)
+        f.argtypes = [vp, fp, fp, fp]
+        f.restype = None
+    lib.fftset_fft_conv_get_kernel.argtypes = [vp, fp, fp]
+    lib.fftset_fft_conv_get_kernel.restype = None
+    for name in ("fftset_fft_conv", "fftset_fft_conv_grad_input", "fftset_fft_conv_grad_kernel"):
+        f = getattr(lib, name)
+        f.argtypes = [vp, fp, fp, fp, fp]
+        f.restype = None
+    lib.fftset_recommend_conv_length.argtypes = [u, u]
+    lib.fftset_recommend_conv_length.restype = u
+    lib.fftset_last_error.argtypes = []
+    lib.fftset_last_error.restype = ctypes.c_int
+    lib.fftset_fft_execute.argtypes = [vp, ctypes.c_int, sz, fp, sz, fp, sz, fp, sz, fp, sz, vp]
+    lib.fftset_fft_execute.restype = ctypes.c_int
+    lib.fftset_fft_complex_bins.argtypes = [vp]
+    lib.fftset_fft_complex_bins.restype = u
+    lib.fftset_fft_radices.argtypes = [vp, ctypes.POINTER(u), u]
+    lib.fftset_fft_radices.restype = u
+    _lib = lib
+    return lib
+
+
+def _modulation(name):
+    lib = load_library()
+    return ctypes.c_void_p.in_dll(lib, name).value
+
+
+def _ptr(x):
+    """Device address of a tensor or an int (None -> NULL)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise ValueError("buffer must be contiguous")
+        if hasattr(x, "dtype") and str(x.dtype) != "torch.float32":
+            raise ValueError("buffer must be float32")
+        return x.data_ptr()
+    raise TypeError(f"not a buffer: {type(x)}")
+
+
+def _check(what):
+    code = load_library().fftset_last_error()
+    if code != OK:
+        raise FFTSetError(code, what)
+
+
+def recommend_conv_length(kernel_length, max_block_size):
+    return int(load_library().fftset_recommend_conv_length(kernel_length, max_block_size))
+
+
+class FFT:
+    """A plan (``const struct fftset_fft *``) owned by an :class:`FFTSet`."""
+
+    def __init__(self, owner, handle, modulation, complex_bins):
+        self._owner = owner
+        self.handle = handle
+        self.modulation = modulation
+        self.complex_bins = complex_bins
+
+    @property
+    def L(self):
+        return self.complex_bins
+
+    def radices(self):
+        buf = (ctypes.c_uint * 64)()
+        n = load_library().fftset_fft_radices(self.handle, buf, 64)
+        return [buf[i] for i in range(n)]
+
+    # drop-in calls (synchronous)
+    def forward(self, out, inp, work):
+        load_library().fftset_fft_forward(self.handle, _ptr(out), _ptr(inp), _ptr(work))
+        _check("forward")
+
+    def inverse(self, out, inp, work):
+        load_library().fftset_fft_inverse(self.handle, _ptr(out), _ptr(inp), _ptr(work))
+        _check("inverse")
+
+    def conv_get_kernel(self, out, inp):
+        load_library().fftset_fft_conv_get_kernel(self.handle, _ptr(out), _ptr(inp))
+        _check("conv_get_kernel")
+
+    def conv(self, out, inp, kern, work):
+        load_library().fftset_fft_conv(self.handle, _ptr(out), _ptr(inp), _ptr(kern), _ptr(work))
+        _check("conv")
+
+    def conv_grad_input(self, grad_in, grad_out, kern, work):
+        load_library().fftset_fft_conv_grad_input(self.handle, _ptr(grad_in), _ptr(grad_out), _ptr(kern),
+                                                  _ptr(work))
+        _check("conv_grad_input")
+
+    def conv_grad_kernel(self, grad_kern, grad_out, inp, work):
+        load_library().fftset_fft_conv_grad_kernel(self.handle, _ptr(grad_kern), _ptr(grad_out), _ptr(inp),
+                                                   _ptr(work))
+        _check("conv_grad_kernel")
+
+    # extension: uniform batch, stream-ordered
+    def execute(self, op, count, out, inp, kern=None, work=None, out_stride=None, in_stride=None,
+                kern_stride=None, work_stride=None, stream=None):
+        n2 = 2 * self.complex_bins
+        if out_stride is None:
+            out_stride = n2
+        if in_stride is None:
+            in_stride = n2
+        if kern_stride is None:
+            kern_stride = n2 if kern is not None else 0
+        if work_stride is None:
+            work_stride = n2 if work is not None else 0
+        if stream is None:
+            try:
+                import torch
+                stream = torch.cuda.current_stream().cuda_stream
+            except Exception:
+                stream = 0
+        rc = load_library().fftset_fft_execute(
+            self.handle, op, count, _ptr(out), out_stride, _ptr(inp), in_stride, _ptr(kern), kern_stride,
+            _ptr(work), work_stride, stream or None)
+        if rc != OK:
+            raise FFTSetError(rc, "execute")
+
+
+class FFTSet:
+    """``struct fftset``: owns plans and the HBM twiddle tables."""
+
+    FREQ_OFFSET_REAL = "FFTSET_MODULATION_FREQ_OFFSET_REAL"
+    COMPLEX = "FFTSET_MODULATION_COMPLEX"
+
+    def __init__(self):
+        lib = load_library()
+        self._mem = (ctypes.c_void_p * 8)()
+        if lib.fftset_init(ctypes.addressof(self._mem)) != 0:
+            raise FFTSetError(E_DEVICE, "fftset_init failed")
+        self._alive = True
+        self._plans = {}
+
+    @property
+    def address(self):
+        return ctypes.addressof(self._mem)
+
+    def create_fft(self, modulation, complex_bins):
+        key = (modulation, int(complex_bins))
+        if key in self._plans:
+            return self._plans[key]
+        lib = load_library()
+        h = lib.fftset_create_fft(self.address, _modulation(modulation), int(complex_bins))
+        if not h:
+            code = lib.fftset_last_error()
+            raise FFTSetError(code, f"create_fft({modulation}, {complex_bins})")
+        p = FFT(self, h, modulation, int(complex_bins))
+        self._plans[key] = p
+        return p
+
+    def destroy(self):
+        if getattr(self, "_alive", False):
+            load_library().fftset_destroy(self.address)
+            self._alive = False
+            self._plans.clear()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+FREQ_OFFSET_REAL = FFTSet.FREQ_OFFSET_REAL
+COMPLEX = FFTSet.COMPLEX

@@ -1,0 +1,406 @@
+// api.cpp -- the fftset C ABI on top of the gfx950 kernels.
+//
+// Mirrors the reference's L3 layer (src/fftset.c): a set object owning plans
+// and twiddle memory, a plan cache keyed by (size, modulation), and four
+// execution calls. Execution is a kernel launch on a per-thread HIP stream
+// followed by a synchronisation, so a drop-in call returns with its result
+// complete, as the reference's CPU call does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "context.h"
+#include "planner.h"
+
+static const fftset_modulation g_mod_fo = {fsa::MOD_FO};
+static const fftset_modulation g_mod_cx = {fsa::MOD_CX};
+
+extern "C" {
+const struct fftset_modulation* FFTSET_MODULATION_FREQ_OFFSET_REAL = &g_mod_fo;
+const struct fftset_modulation* FFTSET_MODULATION_COMPLEX = &g_mod_cx;
+}
+
+namespace fsa {
+
+static thread_local int t_last_error = FFTSET_OK;
+
+static void set_error(int code)
+{
+    if (code != FFTSET_OK)
+        t_last_error = code;
+}
+
+// One non-blocking-free (default-flag) stream per thread and device. Default
+// flags keep it ordered with respect to the legacy default stream.
+static hipStream_t thread_stream(int device)
+{
+    constexpr int kMaxDev = 64;
+    static thread_local hipStream_t streams[kMaxDev] = {};
+    if (device < 0 || device >= kMaxDev)
+        return nullptr;
+    if (!streams[device]) {
+        if (hipStreamCreateWithFlags(&streams[device], hipStreamDefault) != hipSuccess)
+            streams[device] = nullptr;
+    }
+    return streams[device];
+}
+
+static int use_device(const Ctx* ctx)
+{
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess)
+        return FFTSET_E_DEVICE;
+    if (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess)
+        return FFTSET_E_DEVICE;
+    return FFTSET_OK;
+}
+
+// Finds or creates a table of M roots with n | M.
+static const Table* get_table(Ctx* ctx, unsigned n, hipStream_t st)
+{
+    for (const Table& t : ctx->tables)
+        if (t.M % n == 0)
+            return &t;
+    Table t{n, nullptr};
+    if (hipMalloc((void**)&t.dev, sizeof(float2) * (size_t)n) != hipSuccess)
+        return nullptr;
+    if (launch_twiddles(t.dev, (int)n, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipFree(t.dev);
+        return nullptr;
+    }
+    ctx->tables.push_back(t);
+    return &ctx->tables.back();
+}
+
+static int ensure_scratch(Ctx* ctx, size_t floats, float** out)
+{
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->scratch_floats < floats) {
+        if (ctx->scratch) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(ctx->scratch);
+            ctx->scratch = nullptr;
+            ctx->scratch_floats = 0;
+        }
+        if (hipMalloc((void**)&ctx->scratch, floats * sizeof(float)) != hipSuccess)
+            return FFTSET_E_NOMEM;
+        ctx->scratch_floats = floats;
+    }
+    *out = ctx->scratch;
+    return FFTSET_OK;
+}
+
+static int plan_lds(fftset_fft* p, Ctx* ctx, hipStream_t st)
+{
+    const unsigned L = p->lfft;
+    const unsigned tab_n = p->mod == MOD_FO ? 4 * L : L;
+    const Table* tab = get_table(ctx, tab_n, st);
+    if (!tab)
+        return FFTSET_E_NOMEM;
+    p->tw = tab->dev;
+    p->M = tab->M;
+    p->ms = p->mod == MOD_FO ? (int)(tab->M / (4 * L)) : 0;
+    std::vector<int> radices = default_chain(L);
+    if (!build_chain(radices, L, tab->M, &p->chain))
+        return FFTSET_E_UNSUPPORTED;
+    if (!choose_shape(p->chain, 16, 0, &p->shape) && !choose_shape(p->chain, 32, 0, &p->shape))
+        return FFTSET_E_UNSUPPORTED;
+    p->engine = ENG_LDS;
+    return FFTSET_OK;
+}
+
+static int run_lds(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
+                   long long in_st, const float* kern, long long kern_st, hipStream_t st)
+{
+    ShapeD sh = p->shape;
+    if (count < sh.G && !choose_shape(p->chain, sh.E, count, &sh))
+        return FFTSET_E_UNSUPPORTED;
+    LdsArgs a;
+    a.c = p->chain;
+    a.tw = p->tw;
+    a.ms = p->ms;
+    a.L = (int)p->lfft;
+    a.ntrans = count;
+    a.G = sh.G;
+    a.T = sh.T;
+    a.in = in;
+    a.out = out;
+    a.kern = kern;
+    a.in_st = in_st;
+    a.out_st = out_st;
+    a.kern_st = kern_st;
+    return launch_lds(p->mod, op, a, sh, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
+}
+
+static int run_op(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
+                  long long in_st, const float* kern, long long kern_st, hipStream_t st)
+{
+    if (p->engine == ENG_LDS)
+        return run_lds(p, op, count, out, out_st, in, in_st, kern, kern_st, st);
+    return FFTSET_E_UNSUPPORTED;
+}
+
+int execute(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
+            long long in_st, const float* kern, long long kern_st, float* work, long long work_st,
+            hipStream_t st)
+{
+    if (!p || count < 0 || op < 0 || op > FFTSET_OP_CONV_GRAD_KERNEL)
+        return FFTSET_E_INVALID;
+    if (count == 0)
+        return FFTSET_OK;
+    const long long len2 = 2LL * p->lfft;
+    if (!out || !in)
+        return FFTSET_E_INVALID;
+    if ((count > 1 && (out_st < len2 || in_st < len2)) || out_st < 0 || in_st < 0)
+        return FFTSET_E_INVALID;
+    const bool needs_kern = op == FFTSET_OP_CONV || op == FFTSET_OP_CONV_GRAD_INPUT || op == FFTSET_OP_CONV_GRAD_KERNEL;
+    if (needs_kern && (!kern || (kern_st != 0 && kern_st < len2 && count > 1)))
+        return FFTSET_E_INVALID;
+    if (((uintptr_t)out | (uintptr_t)in | (uintptr_t)kern) & 7)
+        return FFTSET_E_INVALID;
+
+    switch (op) {
+    case FFTSET_OP_FORWARD: return run_op(p, OP_FWD, count, out, out_st, in, in_st, nullptr, 0, st);
+    case FFTSET_OP_INVERSE: return run_op(p, OP_INV, count, out, out_st, in, in_st, nullptr, 0, st);
+    case FFTSET_OP_GET_KERNEL: return run_op(p, OP_KERN, count, out, out_st, in, in_st, nullptr, 0, st);
+    case FFTSET_OP_CONV: return run_op(p, OP_CONV, count, out, out_st, in, in_st, kern, kern_st, st);
+    case FFTSET_OP_CONV_GRAD_INPUT: return run_op(p, OP_GRAD_IN, count, out, out_st, in, in_st, kern, kern_st, st);
+    case FFTSET_OP_CONV_GRAD_KERNEL: {
+        // spectrum of the signal into work, then the signal-adjoint pass of
+        // the output gradient (kern) against it
+        float* w = work;
+        long long wst = work_st;
+        if (!w) {
+            int rc = ensure_scratch(p->ctx, (size_t)(count * len2), &w);
+            if (rc != FFTSET_OK)
+                return rc;
+            wst = len2;
+        } else if (count > 1 && wst < len2) {
+            return FFTSET_E_INVALID;
+        }
+        int rc = run_op(p, OP_KERN, count, w, wst, in, in_st, nullptr, 0, st);
+        if (rc != FFTSET_OK)
+            return rc;
+        return run_op(p, OP_GRAD_IN, count, out, out_st, kern, kern_st, w, wst, st);
+    }
+    }
+    return FFTSET_E_INVALID;
+}
+
+static void sync_call(const fftset_fft* p, int op, float* out, const float* in, const float* kern, float* work)
+{
+    if (!p) {
+        set_error(FFTSET_E_INVALID);
+        return;
+    }
+    int rc = use_device(p->ctx);
+    if (rc != FFTSET_OK) {
+        set_error(rc);
+        return;
+    }
+    hipStream_t st = thread_stream(p->ctx->device);
+    if (!st) {
+        set_error(FFTSET_E_DEVICE);
+        return;
+    }
+    const long long len2 = 2LL * p->lfft;
+    rc = execute(p, op, 1, out, len2, in, len2, kern, len2, work, len2, st);
+    if (hipStreamSynchronize(st) != hipSuccess && rc == FFTSET_OK)
+        rc = FFTSET_E_DEVICE;
+    set_error(rc);
+}
+
+}  // namespace fsa
+
+using namespace fsa;
+
+extern "C" {
+
+int fftset_init(struct fftset* fc)
+{
+    if (!fc)
+        return -1;
+    std::memset(fc, 0, sizeof(*fc));
+    Ctx* ctx = new (std::nothrow) Ctx();
+    if (!ctx)
+        return -1;
+    if (hipGetDevice(&ctx->device) != hipSuccess) {
+        delete ctx;
+        return -1;
+    }
+    fc->impl = ctx;
+    return 0;
+}
+
+void fftset_destroy(struct fftset* fc)
+{
+    if (!fc || !fc->impl)
+        return;
+    Ctx* ctx = (Ctx*)fc->impl;
+    (void)use_device(ctx);
+    (void)hipDeviceSynchronize();
+    for (fftset_fft* p = ctx->first; p;) {
+        fftset_fft* n = p->next;
+        delete p;
+        p = n;
+    }
+    for (Table& t : ctx->tables)
+        (void)hipFree(t.dev);
+    if (ctx->scratch)
+        (void)hipFree(ctx->scratch);
+    delete ctx;
+    fc->impl = nullptr;
+}
+
+const struct fftset_fft* fftset_create_fft(struct fftset* fc, const struct fftset_modulation* modulation,
+                                           unsigned complex_bins)
+{
+    if (!fc || !fc->impl || (modulation != &g_mod_fo && modulation != &g_mod_cx)) {
+        set_error(FFTSET_E_INVALID);
+        return nullptr;
+    }
+    Ctx* ctx = (Ctx*)fc->impl;
+    // search the sorted list (src/fftset.c:103-108)
+    for (fftset_fft* p = ctx->first; p; p = p->next) {
+        if (p->lfft == complex_bins && p->modulator == modulation)
+            return p;
+        if (p->lfft < complex_bins)
+            break;
+    }
+    int a, b, c;
+    if (!factor235(complex_bins, &a, &b, &c)) {
+        set_error(FFTSET_E_UNSUPPORTED);
+        return nullptr;
+    }
+    if (use_device(ctx) != FFTSET_OK) {
+        set_error(FFTSET_E_DEVICE);
+        return nullptr;
+    }
+    hipStream_t st = thread_stream(ctx->device);
+    fftset_fft* p = new (std::nothrow) fftset_fft();
+    if (!p || !st) {
+        delete p;
+        set_error(p ? FFTSET_E_DEVICE : FFTSET_E_NOMEM);
+        return nullptr;
+    }
+    p->lfft = complex_bins;
+    p->modulator = modulation;
+    p->ctx = ctx;
+    p->mod = modulation->kind;
+    int rc = fits_lds(complex_bins) ? plan_lds(p, ctx, st) : FFTSET_E_UNSUPPORTED;
+    if (rc != FFTSET_OK) {
+        delete p;
+        set_error(rc);
+        return nullptr;
+    }
+    // insert, keeping descending size (src/fftset.c:122-127)
+    fftset_fft** ipos = &ctx->first;
+    while (*ipos && complex_bins < (*ipos)->lfft) ipos = &(*ipos)->next;
+    p->next = *ipos;
+    *ipos = p;
+    return p;
+}
+
+void fftset_fft_forward(const struct fftset_fft* fft, float* out, const float* in, float* work)
+{
+    sync_call(fft, FFTSET_OP_FORWARD, out, in, nullptr, work);
+}
+
+void fftset_fft_inverse(const struct fftset_fft* fft, float* out, const float* in, float* work)
+{
+    sync_call(fft, FFTSET_OP_INVERSE, out, in, nullptr, work);
+}
+
+void fftset_fft_conv_get_kernel(const struct fftset_fft* fft, float* out, const float* in)
+{
+    sync_call(fft, FFTSET_OP_GET_KERNEL, out, in, nullptr, nullptr);
+}
+
+void fftset_fft_conv(const struct fftset_fft* fft, float* out, const float* in, const float* kern, float* work)
+{
+    sync_call(fft, FFTSET_OP_CONV, out, in, kern, work);
+}
+
+void fftset_fft_conv_grad_input(const struct fftset_fft* fft, float* grad_in, const float* grad_out,
+                                const float* kern, float* work)
+{
+    sync_call(fft, FFTSET_OP_CONV_GRAD_INPUT, grad_in, grad_out, kern, work);
+}
+
+void fftset_fft_conv_grad_kernel(const struct fftset_fft* fft, float* grad_kern, const float* grad_out,
+                                 const float* in, float* work)
+{
+    // execute() reads the signal through `in` and the output gradient
+    // through `kern` for this operation
+    sync_call(fft, FFTSET_OP_CONV_GRAD_KERNEL, grad_kern, in, grad_out, work);
+}
+
+int fftset_fft_execute(const struct fftset_fft* fft, int op, size_t count, float* out, size_t out_stride,
+                       const float* in, size_t in_stride, const float* kern, size_t kern_stride, float* work,
+                       size_t work_stride, void* stream)
+{
+    if (!fft)
+        return FFTSET_E_INVALID;
+    int rc = use_device(fft->ctx);
+    if (rc == FFTSET_OK)
+        rc = execute(fft, op, (long long)count, out, (long long)out_stride, in, (long long)in_stride, kern,
+                     (long long)kern_stride, work, (long long)work_stride, (hipStream_t)stream);
+    set_error(rc);
+    return rc;
+}
+
+int fftset_last_error(void)
+{
+    int e = t_last_error;
+    t_last_error = FFTSET_OK;
+    return e;
+}
+
+unsigned fftset_fft_complex_bins(const struct fftset_fft* fft) { return fft ? fft->lfft : 0; }
+
+unsigned fftset_fft_radices(const struct fftset_fft* fft, unsigned* radices, unsigned max)
+{
+    if (!fft || !radices)
+        return 0;
+    unsigned n = 0;
+    auto put = [&](const ChainD& c) {
+        for (int i = 0; i < c.np && n < max; ++i) radices[n++] = (unsigned)c.p[i].R;
+    };
+    if (fft->engine == ENG_LDS) {
+        put(fft->chain);
+    } else {
+        put(fft->fs.colc);
+        if (n < max)
+            radices[n++] = 0;
+        put(fft->fs.rowc);
+    }
+    return n;
+}
+
+// src/fftset.c:131-167, bit-exact.
+static unsigned round_up_factorisation(unsigned min)
+{
+    unsigned length = 1;
+    while (length < min) {
+        if (length * 2 >= min)
+            length *= 2;
+        else if (length * 3 >= min)
+            length *= 3;
+        else
+            length *= 4;
+    }
+    return length;
+}
+
+unsigned fftset_recommend_conv_length(unsigned kernel_length, unsigned max_block_size)
+{
+    const unsigned kMultiple = 32;  // real-length multiple (src/fftset.c:34)
+    const unsigned target = 8 * kernel_length;
+    const unsigned block = max_block_size > target ? max_block_size : target;
+    const unsigned min_real = kernel_length + block - 1;
+    return (kMultiple / 2) * round_up_factorisation((min_real + kMultiple - 1) / kMultiple);
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// context.h -- host-side objects behind the opaque C types.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "fftset.h"
+#include "kernels.h"
+#include "plan_desc.h"
+
+struct fftset_modulation {
+    int kind;  // fsa::MOD_FO or fsa::MOD_CX
+};
+
+namespace fsa {
+
+// A table of M-th roots of unity in HBM, W_M^k = tw[k]; shared by every plan
+// whose sizes divide M (reference: per-pass tables shared by suffix reuse,
+// src/fftset_vec.c:1064-1120).
+struct Table {
+    unsigned M;
+    float2* dev;
+};
+
+// Four-step decomposition n = A * B for transforms larger than LDS.
+// Column step: B transforms of length A (stride B); row step: A transforms
+// of length B (contiguous), with W_n^{n2 k1} applied between them.
+struct FourStep {
+    unsigned A, B;
+    ChainD colc, rowc;    // chains of the column (length A) and row (length B) transforms
+    ShapeD cols, rows;    // launch shapes of the column and row kernels
+    int tw_n_stride;      // W_n^x = tw[x * tw_n_stride]
+};
+
+struct Ctx {
+    int device = 0;
+    std::vector<Table> tables;
+    fftset_fft* first = nullptr;  // plans sorted by descending size (src/fftset.c:97-129)
+    std::mutex mu;                // guards scratch
+    float* scratch = nullptr;     // library-owned work space for batched calls
+    size_t scratch_floats = 0;
+};
+
+enum Engine : int { ENG_LDS = 0, ENG_FOUR_STEP = 1 };
+
+}  // namespace fsa
+
+struct fftset_fft {
+    // Search keys, as in the reference (src/fftset_modulation.h:32-37).
+    unsigned lfft;
+    const fftset_modulation* modulator;
+    fftset_fft* next;
+
+    fsa::Ctx* ctx;
+    int mod;
+    int engine;
+    fsa::ChainD chain;  // LDS engine chain (ENG_LDS)
+    fsa::ShapeD shape;  // LDS engine launch shape for large batches
+    fsa::FourStep fs;   // ENG_FOUR_STEP
+    const float2* tw;
+    unsigned M;
+    int ms;  // FREQOFFSETREAL modulation stride into tw
+};
+
+namespace fsa {
+// Runs `op` over a uniform batch on `stream` (asynchronous). Returns an
+// FFTSET_* code.
+int execute(const fftset_fft* fft, int op, long long count, float* out, long long out_st, const float* in,
+            long long in_st, const float* kern, long long kern_st, float* work, long long work_st,
+            hipStream_t stream);
+}  // namespace fsa

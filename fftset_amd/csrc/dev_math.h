@@ -1,0 +1,214 @@
+// dev_math.h -- complex helpers and in-register small DFTs for gfx950.
+//
+// Every DFT here is a forward transform X[k] = sum_n x[n] e^{-2 pi i nk/R}
+// on an array held in VGPRs, natural order in and out. Radices 2, 3, 4, 5 are
+// written out; composite radices (6, 8, 9, 10, 12, 15, 16) are one
+// Cooley-Tukey step over them with compile-time twiddle constants, so a whole
+// composite butterfly stays in registers. (The reference builds its radix
+// 6/8/16 butterflies from the same 2/3/4 pieces: src/fftset_vec.c:536-845.)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+#include <utility>
+
+namespace fsa {
+
+// ---------------------------------------------------------------- complex ops
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// -i * a and +i * a
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+__device__ __forceinline__ float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
+
+// ------------------------------------------------------- compile-time loops
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// --------------------------------------------- compile-time roots of unity
+// cos/sin(2 pi p / R) by a Taylor series after shifting the angle into
+// [-pi, pi); evaluated only in constant expressions, rounded once to float.
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double ct_series_sin(double x)
+{
+    double t = x, s = x;
+    for (int k = 1; k < 30; ++k) {
+        t *= -x * x / double((2 * k) * (2 * k + 1));
+        s += t;
+    }
+    return s;
+}
+constexpr double ct_series_cos(double x)
+{
+    double t = 1.0, s = 1.0;
+    for (int k = 1; k < 30; ++k) {
+        t *= -x * x / double((2 * k - 1) * (2 * k));
+        s += t;
+    }
+    return s;
+}
+constexpr double ct_angle(long p, long r)
+{
+    long q = ((p % r) + r) % r;
+    if (2 * q > r)
+        q -= r;  // angle in (-pi, pi]
+    return 2.0 * kPi * double(q) / double(r);
+}
+
+// W_R^P = exp(-2 pi i P / R)
+template <int P, int R>
+struct Root {
+    static constexpr float re = float(ct_series_cos(ct_angle(P, R)));
+    static constexpr float im = float(-ct_series_sin(ct_angle(P, R)));
+};
+
+// x * W_R^P with exact handling of the quarter turns
+template <int P, int R>
+__device__ __forceinline__ float2 rootmul(float2 x)
+{
+    constexpr int p = ((P % R) + R) % R;
+    if constexpr (p == 0) {
+        return x;
+    } else if constexpr (4 * p == R) {
+        return mul_mi(x);
+    } else if constexpr (2 * p == R) {
+        return make_float2(-x.x, -x.y);
+    } else if constexpr (4 * p == 3 * R) {
+        return mul_pi(x);
+    } else if constexpr (8 * p == R) {
+        constexpr float h = Root<1, 8>::re;  // sqrt(1/2)
+        return make_float2(h * (x.x + x.y), h * (x.y - x.x));
+    } else if constexpr (8 * p == 3 * R) {
+        constexpr float h = Root<1, 8>::re;
+        return make_float2(h * (x.y - x.x), -h * (x.x + x.y));
+    } else {
+        return cmul(x, make_float2(Root<p, R>::re, Root<p, R>::im));
+    }
+}
+
+// --------------------------------------------------------------- butterflies
+template <int R>
+struct DFT;
+
+template <>
+struct DFT<1> {
+    static __device__ __forceinline__ void run(float2*) {}
+};
+
+template <>
+struct DFT<2> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <>
+struct DFT<3> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        constexpr float s3 = 0.86602540378443864676f;  // sin(2 pi / 3)
+        float2 t = cadd(v[1], v[2]);
+        float2 d = csub(v[1], v[2]);
+        float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+        float2 s = cscale(d, s3);
+        v[0] = cadd(v[0], t);
+        v[1] = make_float2(m.x + s.y, m.y - s.x);  // m - i s
+        v[2] = make_float2(m.x - s.y, m.y + s.x);  // m + i s
+    }
+};
+
+template <>
+struct DFT<4> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+        float2 c = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+        v[0] = cadd(a, c);
+        v[2] = csub(a, c);
+        v[1] = cadd(b, mul_mi(d));
+        v[3] = csub(b, mul_mi(d));
+    }
+};
+
+template <>
+struct DFT<5> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+        constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+        constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+        constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+        float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+        float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(a1, mul_mi(b1));
+        v[4] = csub(a1, mul_mi(b1));
+        v[2] = cadd(a2, mul_mi(b2));
+        v[3] = csub(a2, mul_mi(b2));
+    }
+};
+
+// One Cooley-Tukey step in registers, R = A*B:
+// X[k1 + A k2] = sum_n2 W_B^{n2 k2} W_R^{n2 k1} sum_n1 x[B n1 + n2] W_A^{n1 k1}
+template <int A, int B>
+struct DFTct {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 t[A * B];
+        static_for<B>([&](auto N2) {
+            constexpr int n2 = decltype(N2)::value;
+            float2 c[A];
+            static_for<A>([&](auto N1) { c[N1] = v[B * decltype(N1)::value + n2]; });
+            DFT<A>::run(c);
+            static_for<A>([&](auto K1) {
+                constexpr int k1 = decltype(K1)::value;
+                t[k1 * B + n2] = rootmul<n2 * k1, A * B>(c[k1]);
+            });
+        });
+        static_for<A>([&](auto K1) {
+            constexpr int k1 = decltype(K1)::value;
+            float2 r[B];
+            static_for<B>([&](auto N2) { r[N2] = t[k1 * B + decltype(N2)::value]; });
+            DFT<B>::run(r);
+            static_for<B>([&](auto K2) { v[k1 + A * decltype(K2)::value] = r[K2]; });
+        });
+    }
+};
+
+template <> struct DFT<6> : DFTct<2, 3> {};
+template <> struct DFT<8> : DFTct<2, 4> {};
+template <> struct DFT<9> : DFTct<3, 3> {};
+template <> struct DFT<10> : DFTct<2, 5> {};
+template <> struct DFT<12> : DFTct<4, 3> {};
+template <> struct DFT<15> : DFTct<3, 5> {};
+template <> struct DFT<16> : DFTct<4, 4> {};
+
+}  // namespace fsa

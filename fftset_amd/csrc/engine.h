@@ -1,0 +1,338 @@
+// engine.h -- LDS-resident mixed-radix passes for gfx950 (wave64).
+//
+// A "tile" is one transform of length n living in LDS; T threads (lanes)
+// work on it. Two pass families are provided, both following the index maps
+// of the reference's CPU passes so the data orders (and the opaque kernel
+// spectrum layout) are well defined:
+//
+//  * Stockham DIF (self-sorting; natural order in, natural order out) for the
+//    forward and inverse transforms -- src/fftset_vec.c:54-63, 90-106:
+//      out[b m + j + k m s] = W_{rm}^{jk} DFT_r(in[b r m + j + q m])[k]
+//    Each pass stages its butterflies in VGPRs, so one LDS array suffices.
+//  * In-place DIF then in-place DIT (no reordering) for convolution, with the
+//    pointwise product fused at the bottom -- src/fftset_vec.c:46-53, 66-89,
+//    1161-1220: positions b r m + j + q m are read and written back.
+//
+// The first pass reads global memory through a Load functor and the last
+// pass writes global memory through a Store functor, so modulation, bin
+// interleave and conjugation are fused into the passes that touch HBM.
+//
+// Twiddles W_N^x come from a shared FP32 table of M-th roots of unity
+// (N divides M): W_N^x = tw[x * (M / N)].
+#pragma once
+
+#include "dev_math.h"
+
+namespace fsa {
+
+// ------------------------------------------------------------ LDS addressing
+// One padding slot every 16 complex values breaks the power-of-two strides of
+// the late passes across the 64 LDS banks.
+__device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 4) + 1; }
+
+struct TileContig {
+    float2* base;
+    __device__ __forceinline__ float2& at(int e) const { return base[lds_pad(e)]; }
+};
+
+// Element e of tile g at lds[e * C + g]: used when lanes run across tiles.
+struct TileInter {
+    float2* lds;
+    int C;
+    __device__ __forceinline__ float2& at(int e) const { return lds[e * C]; }
+};
+
+// ------------------------------------------------------------ pass functions
+// NB = butterflies a thread may own in one pass; u = lt + q*T for q < NB.
+// The host sizes T so that T*NB >= butterflies of every pass.
+
+// First pass (s = 1, m = n/R): reads x[j + q m] through `ld`, writes the
+// tile at j + k m. Shared by Stockham and in-place DIF.
+template <int R, int NB, class Tile, class Load>
+__device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m, const float2* __restrict__ tw,
+                                           int twst, const Load& ld)
+{
+    float2 v[NB][R];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < m) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[q][t] = ld(u + t * m);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < m) {
+            DFT<R>::run(v[q]);
+#pragma unroll
+            for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[u * k * twst]);
+#pragma unroll
+            for (int k = 0; k < R; ++k) tl.at(u + k * m) = v[q][k];
+        }
+    }
+}
+
+// Middle Stockham pass, LDS to LDS through registers.
+template <int R, int NB, class Tile>
+__device__ __forceinline__ void stock_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw,
+                                          int twst)
+{
+    float2 v[NB][R];
+    int bq[NB], jq[NB];
+    const int nbf = m * s;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < nbf) {
+            const int b = u / m, j = u - b * m;
+            bq[q] = b;
+            jq[q] = j;
+            const int base = b * R * m + j;
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[q][t] = tl.at(base + t * m);
+            DFT<R>::run(v[q]);
+#pragma unroll
+            for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[j * k * twst]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < nbf) {
+            const int o = bq[q] * m + jq[q];
+#pragma unroll
+            for (int k = 0; k < R; ++k) tl.at(o + k * m * s) = v[q][k];
+        }
+    }
+}
+
+// Last Stockham pass (m = 1, s = n/R): tile -> `st(f, value)` in natural
+// order f = b + k s.
+template <int R, int NB, class Tile, class Store>
+__device__ __forceinline__ void stock_last(const Tile& tl, int lt, int T, int s, const Store& st)
+{
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int b = lt + q * T;
+        if (b < s) {
+            float2 v[R];
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[t] = tl.at(b * R + t);
+            DFT<R>::run(v);
+#pragma unroll
+            for (int k = 0; k < R; ++k) st(b + k * s, v[k]);
+        }
+    }
+}
+
+// Whole transform in one butterfly (n == R): global -> registers -> global.
+template <int R, class Load, class Store>
+__device__ __forceinline__ void single_pass(int lt, const Load& ld, const Store& st)
+{
+    if (lt == 0) {
+        float2 v[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = ld(t);
+        DFT<R>::run(v);
+#pragma unroll
+        for (int k = 0; k < R; ++k) st(k, v[k]);
+    }
+}
+
+// Middle in-place DIF pass: positions b R m + j + q m, twiddle after the DFT.
+template <int R, int NB, class Tile>
+__device__ __forceinline__ void dif_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw,
+                                        int twst)
+{
+    const int nbf = m * s;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < nbf) {
+            const int b = u / m, j = u - b * m;
+            const int base = b * R * m + j;
+            float2 v[R];
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[t] = tl.at(base + t * m);
+            DFT<R>::run(v);
+#pragma unroll
+            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], tw[j * k * twst]);
+#pragma unroll
+            for (int k = 0; k < R; ++k) tl.at(base + k * m) = v[k];
+        }
+    }
+}
+
+// Middle in-place DIT pass: forward twiddle before a forward DFT (the
+// conjugate-data inverse of src/fftset_vec.c:78-89).
+template <int R, int NB, class Tile>
+__device__ __forceinline__ void dit_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw,
+                                        int twst)
+{
+    const int nbf = m * s;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < nbf) {
+            const int b = u / m, j = u - b * m;
+            const int base = b * R * m + j;
+            float2 v[R];
+            v[0] = tl.at(base);
+#pragma unroll
+            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(base + t * m), tw[j * t * twst]);
+            DFT<R>::run(v);
+#pragma unroll
+            for (int k = 0; k < R; ++k) tl.at(base + k * m) = v[k];
+        }
+    }
+}
+
+// Last DIT pass (s = 1, m = n/R): tile -> store in natural order j + k m.
+template <int R, int NB, class Tile, class Store>
+__device__ __forceinline__ void dit_last(const Tile& tl, int lt, int T, int m, const float2* __restrict__ tw,
+                                         int twst, const Store& st)
+{
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < m) {
+            float2 v[R];
+            v[0] = tl.at(u);
+#pragma unroll
+            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(u + t * m), tw[u * t * twst]);
+            DFT<R>::run(v);
+#pragma unroll
+            for (int k = 0; k < R; ++k) st(u + k * m, v[k]);
+        }
+    }
+}
+
+// Bottom of the in-place DIF chain (m = 1, s = n/R): the R outputs of block
+// b are handed to `bot(b, v)` while still in registers. The kernel spectrum
+// layout used by the bottom operators is slot-major: value k of block b at
+// kern[k * s + b] (coalesced across lanes).
+template <int R, int NB, class Tile, class Bottom>
+__device__ __forceinline__ void dif_bottom(const Tile& tl, int lt, int T, int s, const Bottom& bot)
+{
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int b = lt + q * T;
+        if (b < s) {
+            float2 v[R];
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[t] = tl.at(b * R + t);
+            DFT<R>::run(v);
+            bot.template apply<R>(tl, b, s, v);
+        }
+    }
+}
+
+// ------------------------------------------------------------ bottom operators
+// Convolution: w <- conj(w * K) (src/fftset_vec.c:847-867), then the first
+// DIT pass (m = 1: a plain forward DFT) in registers, back to the tile.
+struct BotConv {
+    const float2* __restrict__ kern;
+    bool conj_kern;  // true: w <- conj(w * conj(K)) (adjoint w.r.t. the signal)
+    template <int R, class Tile>
+    __device__ __forceinline__ void apply(const Tile& tl, int b, int s, float2* v) const
+    {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float2 K = kern[k * s + b];
+            v[k] = cconj(conj_kern ? cmulc(v[k], K) : cmul(v[k], K));
+        }
+        DFT<R>::run(v);
+#pragma unroll
+        for (int t = 0; t < R; ++t) tl.at(b * R + t) = v[t];
+    }
+};
+
+// Kernel spectrum: written out slot-major.
+struct BotKernOut {
+    float2* __restrict__ out;
+    template <int R, class Tile>
+    __device__ __forceinline__ void apply(const Tile&, int b, int s, float2* v) const
+    {
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[k * s + b] = v[k];
+    }
+};
+
+// ------------------------------------------------------------ I/O functors
+// FREQOFFSETREAL first pass (src/fftset_mod_freqoffsetreal.c:917-924):
+// z[e] = (x[e] - i x[L+e]) e^{-i pi e / (2L)}, modulation from the table.
+struct LdFoMod {
+    const float* __restrict__ x;
+    int L;
+    const float2* __restrict__ tw;
+    int ms;
+    __device__ __forceinline__ float2 operator()(int e) const
+    {
+        const float re = x[e], im = x[L + e];
+        const float2 w = tw[e * ms];
+        return make_float2(re * w.x + im * w.y, re * w.y - im * w.x);
+    }
+};
+
+// FREQOFFSETREAL inverse input (src/fftset_mod_freqoffsetreal.c:985-998):
+// w[e] = conj X[2e] for e < ceil(L/2), else X[2(L-1-e)+1].
+struct LdFoBins {
+    const float2* __restrict__ in;
+    int L;
+    __device__ __forceinline__ float2 operator()(int e) const
+    {
+        return (e < ((L + 1) >> 1)) ? cconj(in[2 * e]) : in[2 * (L - 1 - e) + 1];
+    }
+};
+
+template <bool CONJ>
+struct LdCx {
+    const float2* __restrict__ in;
+    __device__ __forceinline__ float2 operator()(int e) const
+    {
+        const float2 v = in[e];
+        return CONJ ? cconj(v) : v;
+    }
+};
+
+// FREQOFFSETREAL forward output (src/fftset_mod_freqoffsetreal.c:958-971):
+// bin 2p <- Z[p], bin 2p+1 <- conj Z[L-1-p].
+struct StFoBins {
+    float2* __restrict__ out;
+    int L;
+    __device__ __forceinline__ void operator()(int f, float2 v) const
+    {
+        if (f < ((L + 1) >> 1))
+            out[2 * f] = v;
+        else
+            out[2 * (L - 1 - f) + 1] = cconj(v);
+    }
+};
+
+// FREQOFFSETREAL last pass of the inverse and of the convolution
+// (src/fftset_mod_freqoffsetreal.c:926-933, 1002-1009):
+// y[f] = Re(v W), y[L+f] = Im(v W), W = e^{-i pi f / (2L)}.
+struct StFoDemod {
+    float* __restrict__ y;
+    int L;
+    const float2* __restrict__ tw;
+    int ms;
+    __device__ __forceinline__ void operator()(int f, float2 v) const
+    {
+        const float2 w = tw[f * ms];
+        y[f] = v.x * w.x - v.y * w.y;
+        y[L + f] = v.x * w.y + v.y * w.x;
+    }
+};
+
+template <bool CONJ>
+struct StCx {
+    float2* __restrict__ out;
+    __device__ __forceinline__ void operator()(int f, float2 v) const { out[f] = CONJ ? cconj(v) : v; }
+};
+
+}  // namespace fsa

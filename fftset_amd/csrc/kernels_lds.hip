@@ -1,0 +1,249 @@
+// kernels_lds.hip -- single-kernel LDS-resident transforms (n fits in LDS)
+// and the FP64 twiddle generator.
+//
+// One workgroup holds G transforms of one plan in LDS, T lanes each. The
+// radix chain is a runtime descriptor; every pass dispatches on its radix to
+// a fully unrolled register butterfly. Modulation, bin interleave,
+// conjugation and the convolution product are fused into the passes that
+// read or write HBM, so each transform moves exactly its input, output and
+// (for convolution) kernel spectrum through HBM once.
+#include "kernels.h"
+
+namespace fsa {
+
+// Radix dispatch: BODY sees constexpr RR (radix) and NB (butterflies per
+// thread for the element budget E).
+#define FSA_RADIX_CASE(RV, E, BODY)                                   \
+    case RV: {                                                        \
+        constexpr int RR = RV;                                        \
+        constexpr int NB = ((E) / RV) > 0 ? ((E) / RV) : 1;           \
+        (void)NB;                                                     \
+        BODY;                                                         \
+    } break;
+#define FSA_RADIX_SWITCH(R, E, BODY)                                  \
+    switch (R) {                                                      \
+        FSA_RADIX_CASE(2, E, BODY)                                    \
+        FSA_RADIX_CASE(3, E, BODY)                                    \
+        FSA_RADIX_CASE(4, E, BODY)                                    \
+        FSA_RADIX_CASE(5, E, BODY)                                    \
+        FSA_RADIX_CASE(6, E, BODY)                                    \
+        FSA_RADIX_CASE(8, E, BODY)                                    \
+        FSA_RADIX_CASE(9, E, BODY)                                    \
+        FSA_RADIX_CASE(10, E, BODY)                                   \
+        FSA_RADIX_CASE(12, E, BODY)                                   \
+        FSA_RADIX_CASE(15, E, BODY)                                   \
+        FSA_RADIX_CASE(16, E, BODY)                                   \
+        default: break;                                               \
+    }
+
+// ---------------------------------------------------------------- drivers
+template <int R, class Load, class Store>
+__device__ __forceinline__ void single_conv(int lt, const Load& ld, const Store& st, const BotConv& bot)
+{
+    if (lt == 0) {
+        float2 v[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = ld(t);
+        DFT<R>::run(v);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float2 K = bot.kern[k];
+            v[k] = cconj(bot.conj_kern ? cmulc(v[k], K) : cmul(v[k], K));
+        }
+        DFT<R>::run(v);
+#pragma unroll
+        for (int k = 0; k < R; ++k) st(k, v[k]);
+    }
+}
+
+template <int R, class Load>
+__device__ __forceinline__ void single_kern(int lt, const Load& ld, float2* out)
+{
+    if (lt == 0) {
+        float2 v[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = ld(t);
+        DFT<R>::run(v);
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[k] = v[k];
+    }
+}
+
+// Stockham chain: natural order in (through ld), natural order out (st).
+template <int E, class Tile, class Load, class Store>
+__device__ __forceinline__ void run_stockham(const ChainD& c, const float2* tw, const Tile& tl, int lt, int T,
+                                             const Load& ld, const Store& st)
+{
+    const int np = c.np;
+    if (np == 1) {
+        FSA_RADIX_SWITCH(c.p[0].R, E, (single_pass<RR>(lt, ld, st)));
+        return;
+    }
+    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, ld)));
+    __syncthreads();
+    for (int i = 1; i < np - 1; ++i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (stock_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        __syncthreads();
+    }
+    FSA_RADIX_SWITCH(c.p[np - 1].R, E, (stock_last<RR, NB>(tl, lt, T, c.p[np - 1].s, st)));
+}
+
+// In-place DIF chain, product at the bottom, in-place DIT chain back.
+template <int E, class Tile, class Load, class Store>
+__device__ __forceinline__ void run_conv(const ChainD& c, const float2* tw, const Tile& tl, int lt, int T,
+                                         const Load& ld, const Store& st, const BotConv& bot)
+{
+    const int np = c.np;
+    if (np == 1) {
+        FSA_RADIX_SWITCH(c.p[0].R, E, (single_conv<RR>(lt, ld, st, bot)));
+        return;
+    }
+    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, ld)));
+    __syncthreads();
+    for (int i = 1; i < np - 1; ++i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        __syncthreads();
+    }
+    FSA_RADIX_SWITCH(c.p[np - 1].R, E, (dif_bottom<RR, NB>(tl, lt, T, c.p[np - 1].s, bot)));
+    __syncthreads();
+    for (int i = np - 2; i >= 1; --i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dit_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        __syncthreads();
+    }
+    FSA_RADIX_SWITCH(c.p[0].R, E, (dit_last<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, st)));
+}
+
+// In-place DIF chain whose bottom writes the kernel spectrum.
+template <int E, class Tile, class Load>
+__device__ __forceinline__ void run_kern(const ChainD& c, const float2* tw, const Tile& tl, int lt, int T,
+                                         const Load& ld, float2* out)
+{
+    const int np = c.np;
+    if (np == 1) {
+        FSA_RADIX_SWITCH(c.p[0].R, E, (single_kern<RR>(lt, ld, out)));
+        return;
+    }
+    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, ld)));
+    __syncthreads();
+    for (int i = 1; i < np - 1; ++i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        __syncthreads();
+    }
+    const BotKernOut bot{out};
+    FSA_RADIX_SWITCH(c.p[np - 1].R, E, (dif_bottom<RR, NB>(tl, lt, T, c.p[np - 1].s, bot)));
+}
+
+// ---------------------------------------------------------------- kernel
+// E = 16 keeps a butterfly set within 128 VGPRs (1024-lane workgroups);
+// E = 32 needs up to 256 and is launched with at most 512 lanes.
+template <int E>
+constexpr int lds_max_threads() { return E <= 16 ? 1024 : 512; }
+
+template <int MOD, int OP, int E>
+__global__ void __launch_bounds__(lds_max_threads<E>()) k_lds(const LdsArgs a)
+{
+    extern __shared__ float2 smem[];
+    const int T = a.T;
+    const int g = threadIdx.x / T;
+    const long long tr = (long long)blockIdx.x * a.G + g;
+    const bool valid = (g < a.G) && (tr < a.ntrans);
+    const int lt = valid ? (int)(threadIdx.x - g * T) : (1 << 28);
+    const long long t0 = valid ? tr : 0;
+    const TileContig tl{smem + (valid ? g : 0) * lds_len(a.L)};
+    const float* in = a.in + t0 * a.in_st;
+    float* out = a.out + t0 * a.out_st;
+    const int L = a.L;
+
+    if constexpr (OP == OP_FWD) {
+        if constexpr (MOD == MOD_FO)
+            run_stockham<E>(a.c, a.tw, tl, lt, T, LdFoMod{in, L, a.tw, a.ms}, StFoBins{(float2*)out, L});
+        else
+            run_stockham<E>(a.c, a.tw, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<false>{(float2*)out});
+    } else if constexpr (OP == OP_INV) {
+        if constexpr (MOD == MOD_FO)
+            run_stockham<E>(a.c, a.tw, tl, lt, T, LdFoBins{(const float2*)in, L}, StFoDemod{out, L, a.tw, a.ms});
+        else
+            run_stockham<E>(a.c, a.tw, tl, lt, T, LdCx<true>{(const float2*)in}, StCx<true>{(float2*)out});
+    } else if constexpr (OP == OP_KERN) {
+        if constexpr (MOD == MOD_FO)
+            run_kern<E>(a.c, a.tw, tl, lt, T, LdFoMod{in, L, a.tw, a.ms}, (float2*)out);
+        else
+            run_kern<E>(a.c, a.tw, tl, lt, T, LdCx<false>{(const float2*)in}, (float2*)out);
+    } else {
+        const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN};
+        if constexpr (MOD == MOD_FO)
+            run_conv<E>(a.c, a.tw, tl, lt, T, LdFoMod{in, L, a.tw, a.ms}, StFoDemod{out, L, a.tw, a.ms}, bot);
+        else
+            run_conv<E>(a.c, a.tw, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<true>{(float2*)out}, bot);
+    }
+}
+
+// ---------------------------------------------------------------- twiddles
+// tw[k] = exp(-2 pi i k / M), evaluated in FP64 and rounded once to FP32.
+// This is the only FP64 arithmetic in the library.
+__global__ void k_twiddle_fp64(float2* __restrict__ tw, int M)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M)
+        return;
+    const long long k4 = 4LL * k;
+    if (k4 % M == 0) {
+        const int q = (int)(k4 / M);  // quarter turns are exact
+        const float c[4] = {1.f, 0.f, -1.f, 0.f};
+        const float s[4] = {0.f, -1.f, 0.f, 1.f};
+        tw[k] = make_float2(c[q], s[q]);
+        return;
+    }
+    long long r = k;
+    if (2 * r > M)
+        r -= M;  // angle in (-pi, pi]
+    double s, c;
+    sincospi(-2.0 * (double)r / (double)M, &s, &c);
+    tw[k] = make_float2((float)c, (float)s);
+}
+
+hipError_t launch_twiddles(float2* tw, int M, hipStream_t st)
+{
+    const int threads = 256;
+    hipLaunchKernelGGL(k_twiddle_fp64, dim3((M + threads - 1) / threads), dim3(threads), 0, st, tw, M);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- launcher
+template <int MOD, int OP, int E>
+static hipError_t launch_one(const LdsArgs& a, const ShapeD& sh, hipStream_t st)
+{
+    const long long grid = (a.ntrans + sh.G - 1) / sh.G;
+    if (grid <= 0)
+        return hipSuccess;
+    auto fn = k_lds<MOD, OP, E>;
+    if (sh.lds_bytes > 64 * 1024) {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(sh.threads), sh.lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+template <int MOD, int E>
+static hipError_t launch_op(int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st)
+{
+    switch (op) {
+    case OP_FWD: return launch_one<MOD, OP_FWD, E>(a, sh, st);
+    case OP_INV: return launch_one<MOD, OP_INV, E>(a, sh, st);
+    case OP_KERN: return launch_one<MOD, OP_KERN, E>(a, sh, st);
+    case OP_CONV: return launch_one<MOD, OP_CONV, E>(a, sh, st);
+    case OP_GRAD_IN: return launch_one<MOD, OP_GRAD_IN, E>(a, sh, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_lds(int mod, int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st)
+{
+    if (sh.E == 16)
+        return mod == MOD_FO ? launch_op<MOD_FO, 16>(op, a, sh, st) : launch_op<MOD_CX, 16>(op, a, sh, st);
+    if (sh.E == 32)
+        return mod == MOD_FO ? launch_op<MOD_FO, 32>(op, a, sh, st) : launch_op<MOD_CX, 32>(op, a, sh, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fsa

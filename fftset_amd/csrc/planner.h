@@ -1,0 +1,30 @@
+// planner.h -- radix chains and launch shapes (host side).
+#pragma once
+
+#include <vector>
+
+#include "plan_desc.h"
+
+namespace fsa {
+
+// Factors n = 2^a 3^b 5^c; returns false for anything else (or 5^2 and up,
+// which the reference cannot plan either: radix 5 is final-only,
+// src/fftset_vec.c:944-971).
+bool factor235(unsigned n, int* a, int* b, int* c);
+
+// Default chain for the LDS engine: few passes of register-resident radices
+// from {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2}, largest first.
+std::vector<int> default_chain(unsigned n);
+
+// Fills a ChainD for `radices` over length n with twiddle strides for a
+// table of M roots (n must divide M).
+bool build_chain(const std::vector<int>& radices, unsigned n, unsigned M, ChainD* out);
+
+// Lanes per transform / transforms per workgroup for element budget E.
+// `batch_hint` caps G for small batches. Returns false when n does not fit.
+bool choose_shape(const ChainD& c, int E, long long batch_hint, ShapeD* out);
+
+// Largest n the single-kernel LDS engine accepts.
+bool fits_lds(unsigned n);
+
+}  // namespace fsa

@@ -1,0 +1,164 @@
+/* fftset.h -- MI355X (gfx950) implementation of the fftset C API.
+ *
+ * Drop-in for the reference interface in fftset.h of templeblock/fftset:
+ * same names, argument meaning and buffer rules, except that every buffer
+ * argument is a DEVICE pointer (hipMalloc'd memory, or any memory the GPU can
+ * address) aligned to at least 16 bytes. The drop-in execution calls run on a
+ * per-thread HIP stream and return when their results are complete, as the
+ * reference's CPU calls do. Extensions (stream-ordered variants, uniform and
+ * ragged batches, convolution adjoints) follow the drop-in section and have
+ * no counterpart in the reference.
+ *
+ * Sizes: complex_bins = 2^a 3^b 5^{0,1}, complex_bins >= 2 (the reference's
+ * supported set; it leaves other sizes undefined, this library returns NULL).
+ * Every buffer holds 2*complex_bins floats. */
+#ifndef FFTSET_H
+#define FFTSET_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types */
+
+/* Holds the memory for every transform created from it (plans and the
+ * shared FP32 twiddle tables in HBM). A complete type so it can live on the
+ * stack (reference fftset.h:29-31, 159-167); the members are private. */
+struct fftset {
+	void *impl;
+	void *reserved[7];
+};
+
+/* Opaque (reference fftset.h:33-39). */
+struct fftset_modulation;
+struct fftset_fft;
+
+/* ---------------------------------------------- initialisation / cleanup */
+
+/* Replaces fftset_init (reference fftset.h:44-47, src/fftset.c:80-85).
+ * Binds the set to the current HIP device. Returns 0 on success, non-zero
+ * when memory is exhausted or no device is usable. */
+int fftset_init(struct fftset *fc);
+
+/* Replaces fftset_destroy (reference fftset.h:49-52, src/fftset.c:87-95).
+ * Frees every plan and twiddle table; every fftset_fft from this set becomes
+ * invalid. */
+void fftset_destroy(struct fftset *fc);
+
+/* ------------------------------------------------------------ modulations */
+
+/* Replaces FFTSET_MODULATION_FREQ_OFFSET_REAL (reference fftset.h:62-73):
+ *   X[k] = sum_{n<N} x[n] e^{-2 pi i n (k + 1/2) / N},  N = 2*complex_bins,
+ * x real (N floats in), X[k] for k < complex_bins interleaved re/im out.
+ * Its convolution is negacyclic of length N, scaled by complex_bins. */
+extern const struct fftset_modulation *FFTSET_MODULATION_FREQ_OFFSET_REAL;
+
+/* Replaces FFTSET_MODULATION_COMPLEX (reference fftset.h:75-76): a plain
+ * complex DFT of complex_bins points, interleaved re/im, natural order;
+ * inverse unnormalised; convolution cyclic, scaled by complex_bins. */
+extern const struct fftset_modulation *FFTSET_MODULATION_COMPLEX;
+
+/* Replaces fftset_create_fft (reference fftset.h:78, src/fftset.c:97-129).
+ * Returns the existing plan for (modulation, complex_bins) or creates it;
+ * NULL for an unsupported size or when memory is exhausted. Not thread-safe
+ * (as in the reference). */
+const struct fftset_fft *fftset_create_fft(struct fftset *fc, const struct fftset_modulation *modulation,
+                                           unsigned complex_bins);
+
+/* -------------------------------------------------------------- execution
+ * Buffer rules as in the reference (fftset.h:80-108): input and output may
+ * alias; work must alias neither; kernel_buf must alias nothing. Buffers are
+ * device pointers. Errors are reported through fftset_last_error(). */
+
+/* Replaces fftset_fft_forward (reference fftset.h:109-115, src/fftset.c:58-67). */
+void fftset_fft_forward(const struct fftset_fft *first_pass, float *output_buf, const float *input_buf,
+                        float *work_buf);
+
+/* Replaces fftset_fft_inverse (reference fftset.h:117-123, src/fftset.c:69-78). */
+void fftset_fft_inverse(const struct fftset_fft *first_pass, float *output_buf, const float *input_buf,
+                        float *work_buf);
+
+/* Replaces fftset_fft_conv_get_kernel (reference fftset.h:125-130,
+ * src/fftset.c:36-44). The output is an opaque, unnormalised kernel spectrum
+ * of 2*complex_bins floats in this library's own order, valid only for
+ * the same fftset_fft. input_buf and output_buf must not alias. */
+void fftset_fft_conv_get_kernel(const struct fftset_fft *first_pass, float *output_buf, const float *input_buf);
+
+/* Replaces fftset_fft_conv (reference fftset.h:132-139, src/fftset.c:46-56):
+ * FREQOFFSETREAL: out = complex_bins * negacyclic_N(in, h);
+ * COMPLEX:        out = complex_bins * cyclic(in, h),
+ * where kernel_buf = fftset_fft_conv_get_kernel(h). */
+void fftset_fft_conv(const struct fftset_fft *first_pass, float *output_buf, const float *input_buf,
+                     const float *kernel_buf, float *work_buf);
+
+/* Replaces fftset_recommend_conv_length (reference fftset.h:144-151,
+ * src/fftset.c:131-167), bit-exact. The result is in complex bins. */
+unsigned fftset_recommend_conv_length(unsigned kernel_length, unsigned max_block_size);
+
+/* ============================================================ extensions
+ * None of the following has a counterpart in the reference. */
+
+/* Error codes. */
+#define FFTSET_OK               0
+#define FFTSET_E_INVALID       (-1)  /* bad argument (NULL plan, bad op, ...) */
+#define FFTSET_E_NOMEM         (-2)  /* host or device allocation failed      */
+#define FFTSET_E_DEVICE        (-3)  /* a HIP call failed                     */
+#define FFTSET_E_UNSUPPORTED   (-4)  /* size or operation not supported       */
+
+/* Returns the code of the most recent failure on the calling thread and
+ * clears it (FFTSET_OK when nothing failed since the last call). */
+int fftset_last_error(void);
+
+/* Operations for the batched and stream-ordered entry points. */
+#define FFTSET_OP_FORWARD          0
+#define FFTSET_OP_INVERSE          1
+#define FFTSET_OP_GET_KERNEL       2
+#define FFTSET_OP_CONV             3
+#define FFTSET_OP_CONV_GRAD_INPUT  4
+#define FFTSET_OP_CONV_GRAD_KERNEL 5
+
+/* Uniform batch, stream-ordered (returns before completion): `count`
+ * transforms of one plan; transform t reads in + t*in_stride, writes
+ * out + t*out_stride (strides in floats, >= 2*complex_bins unless 0 for a
+ * buffer shared by all transforms, which is allowed for kern only).
+ *   FORWARD / INVERSE / GET_KERNEL: in -> out
+ *   CONV:             out = conv(in, kern)
+ *   CONV_GRAD_INPUT:  out = adjoint of conv w.r.t. the signal applied to
+ *                     in (the output gradient), with kern = get_kernel(h)
+ *   CONV_GRAD_KERNEL: out = adjoint w.r.t. the kernel taps applied to kern
+ *                     (the output gradient) with in = the signal x
+ * work (may be NULL) provides count*work_stride floats of scratch for the
+ * operations that need it; when NULL the library uses its own. stream is a
+ * hipStream_t (NULL = the default stream). Returns FFTSET_OK or an error. */
+int fftset_fft_execute(const struct fftset_fft *fft, int op, size_t count, float *out, size_t out_stride,
+                       const float *in, size_t in_stride, const float *kern, size_t kern_stride, float *work,
+                       size_t work_stride, void *stream);
+
+/* Adjoints of fftset_fft_conv (synchronous, like the drop-in calls).
+ * For FREQOFFSETREAL, with y = conv(x, h) = L * negacyclic(x, h) and an
+ * output gradient g = dLoss/dy:
+ *   grad_input  = dLoss/dx = L * negacyclic(g, h_flat),  h_flat[0] = h[0],
+ *                 h_flat[k] = -h[N-k]  (spectrum: conj of the kernel's)
+ *   grad_kernel = dLoss/dh = L * negacyclic(g, x_flat)
+ * For COMPLEX the same calls apply the conjugate-transpose maps.
+ * kernel_buf = fftset_fft_conv_get_kernel(h); work_buf holds 2L floats and
+ * aliases nothing. */
+void fftset_fft_conv_grad_input(const struct fftset_fft *first_pass, float *grad_input, const float *grad_output,
+                                const float *kernel_buf, float *work_buf);
+void fftset_fft_conv_grad_kernel(const struct fftset_fft *first_pass, float *grad_kernel, const float *grad_output,
+                                 const float *input_buf, float *work_buf);
+
+/* Plan introspection (for tests, benchmarks and the autotuner). */
+unsigned fftset_fft_complex_bins(const struct fftset_fft *fft);
+/* Writes at most `max` radices of the plan's LDS chain; returns the count
+ * (for four-step plans: the chains of the column and row transforms, with
+ * a 0 separating them). */
+unsigned fftset_fft_radices(const struct fftset_fft *fft, unsigned *radices, unsigned max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFTSET_H */

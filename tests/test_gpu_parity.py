@@ -1,0 +1,179 @@
+"""GPU parity: the HIP path through the C ABI against the oracle (the FP32
+restatement of the reference, <= 2e-6 relative RMS per transform), against
+FP64 closed forms (<= 1e-6 * log2(L) relative RMS), and against the
+reference's own known-answer tests with its thresholds."""
+import numpy as np
+import pytest
+
+try:  # import at collection: the first import on a fresh box is slow
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+import fftset_amd
+from oracle import fp64
+from oracle.oracle import MOD_COMPLEX as OCX, MOD_FREQ_OFFSET_REAL as OFO
+from tests import kat
+from tests.gpu_util import (gpu_batch, gpu_conv, gpu_forward, gpu_get_kernel, gpu_inverse, torch_cuda)
+
+pytestmark = pytest.mark.gpu
+
+REF_TOL = 2e-6      # relative RMS against the reference restatement
+
+
+def fp64_tol(L):
+    return 1e-6 * max(1.0, np.log2(L))
+
+
+@pytest.fixture(scope="module")
+def fs():
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    s = fftset_amd.FFTSet()
+    yield s
+    s.destroy()
+
+
+FO, CX = fftset_amd.FREQ_OFFSET_REAL, fftset_amd.COMPLEX
+
+
+# ---------------------------------------------------------------- reference KATs
+@pytest.mark.parametrize("L", kat.TEST_LENGTHS)
+def test_kat_fo_prime_impulse(fs, L):
+    f = fs.create_fft(FO, L)
+    x, X = kat.fo_impulse_case(L)
+    Y = gpu_forward(f, x)
+    assert kat.abs_rms_bins(Y, X, L) <= kat.FWD_RMS
+    assert kat.abs_rms_bins(gpu_inverse(f, Y) / L, x, L) <= kat.INV_RMS
+
+
+@pytest.mark.parametrize("L", kat.TEST_LENGTHS)
+def test_kat_cx_prime_impulse(fs, L):
+    f = fs.create_fft(CX, L)
+    x, X = kat.cx_impulse_case(L)
+    Y = gpu_forward(f, x)
+    assert kat.abs_rms_bins(Y, X, L) <= kat.FWD_RMS
+    assert kat.abs_rms_bins(gpu_inverse(f, Y) / L, x, L) <= kat.INV_RMS
+
+
+@pytest.mark.parametrize("L", kat.TEST_LENGTHS)
+def test_kat_convolution(fs, L):
+    f = fs.create_fft(FO, L)
+    x, h, expect = kat.conv_case(L)
+    assert kat.abs_rms_bins(gpu_conv(f, x, h), expect, L) <= kat.CONV_RMS
+
+
+# ---------------------------------------------------------------- random parity
+SIZES = [2, 3, 5, 6, 9, 10, 12, 15, 16, 18, 27, 45, 48, 64, 96, 120, 128, 240, 360, 512, 576, 768,
+         960, 1440, 2048, 3840, 4096, 4608, 6144, 9216, 15360, 16384]
+
+
+@pytest.mark.parametrize("L", SIZES)
+def test_complex_parity(fs, oracle, L):
+    rng = np.random.default_rng(1000 + L)
+    f = fs.create_fft(CX, L)
+    o = oracle.plan(OCX, L)
+    x = rng.standard_normal(2 * L).astype(np.float32)
+    y = gpu_forward(f, x)
+    assert fp64.rel_rms(y, o.forward(x)) <= REF_TOL
+    assert fp64.rel_rms(y, fp64.cx_forward64(x)) <= fp64_tol(L)
+    z = gpu_inverse(f, x)
+    assert fp64.rel_rms(z, o.inverse(x)) <= REF_TOL
+    assert fp64.rel_rms(z, fp64.cx_inverse64(x)) <= fp64_tol(L)
+    h = rng.standard_normal(2 * L).astype(np.float32)
+    c = gpu_conv(f, x, h)
+    assert fp64.rel_rms(c, o.conv(x, o.get_kernel(h))) <= REF_TOL
+    assert fp64.rel_rms(c, fp64.cx_conv64(x, h)) <= fp64_tol(L)
+
+
+@pytest.mark.parametrize("L", SIZES)
+def test_freq_offset_real_parity(fs, oracle, L):
+    rng = np.random.default_rng(2000 + L)
+    f = fs.create_fft(FO, L)
+    o = oracle.plan(OFO, L)
+    x = rng.uniform(-1, 1, 2 * L).astype(np.float32)
+    X = gpu_forward(f, x)
+    assert fp64.rel_rms(X, o.forward(x)) <= REF_TOL
+    assert fp64.rel_rms(X, fp64.fo_forward64(x)) <= fp64_tol(L)
+    Xr = rng.standard_normal(2 * L).astype(np.float32)
+    y = gpu_inverse(f, Xr)
+    assert fp64.rel_rms(y, o.inverse(Xr)) <= REF_TOL
+    assert fp64.rel_rms(y, fp64.fo_inverse64(Xr)) <= fp64_tol(L)
+    # full-block kernel: the convolution wraps, so it must be negacyclic
+    h = (rng.standard_normal(2 * L) * np.exp(-np.arange(2 * L) / max(1.0, L / 4))).astype(np.float32)
+    c = gpu_conv(f, x, h)
+    assert fp64.rel_rms(c, o.conv(x, o.get_kernel(h))) <= REF_TOL
+    assert fp64.rel_rms(c, fp64.fo_conv64(x, h)) <= fp64_tol(L)
+
+
+@pytest.mark.parametrize("mod", [FO, CX])
+@pytest.mark.parametrize("L", [4, 45, 512, 3840])
+def test_in_place_aliasing(fs, mod, L):
+    # input and output may alias for forward, inverse and conv
+    from tests.gpu_util import dev, empty, host
+    rng = np.random.default_rng(L)
+    f = fs.create_fft(mod, L)
+    x = rng.standard_normal(2 * L).astype(np.float32)
+    h = rng.standard_normal(2 * L).astype(np.float32)
+    ref_f, ref_i, ref_c = gpu_forward(f, x), gpu_inverse(f, x), gpu_conv(f, x, h)
+    work, k = empty(2 * L), empty(2 * L)
+    buf = dev(x)
+    f.forward(buf, buf, work)
+    assert np.array_equal(host(buf), ref_f)
+    buf = dev(x)
+    f.inverse(buf, buf, work)
+    assert np.array_equal(host(buf), ref_i)
+    f.conv_get_kernel(k, dev(h))
+    buf = dev(x)
+    f.conv(buf, buf, k, work)
+    assert np.array_equal(host(buf), ref_c)
+
+
+@pytest.mark.parametrize("mod,L", [(CX, 48), (FO, 120), (CX, 3840), (FO, 4096), (FO, 2048)])
+def test_uniform_batch_matches_single_calls(fs, oracle, mod, L):
+    rng = np.random.default_rng(7 * L)
+    f = fs.create_fft(mod, L)
+    o = oracle.plan(OFO if mod == FO else OCX, L)
+    B = 37
+    x = rng.standard_normal((B, 2 * L)).astype(np.float32)
+    h = rng.standard_normal((B, 2 * L)).astype(np.float32)
+    y = gpu_batch(f, fftset_amd.OP_FORWARD, x)
+    assert np.max(fp64.rel_rms(y, o.forward(x))) <= REF_TOL
+    z = gpu_batch(f, fftset_amd.OP_INVERSE, x)
+    assert np.max(fp64.rel_rms(z, o.inverse(x))) <= REF_TOL
+    K = gpu_batch(f, fftset_amd.OP_GET_KERNEL, h)
+    c = gpu_batch(f, fftset_amd.OP_CONV, x, K)
+    assert np.max(fp64.rel_rms(c, o.conv(x, o.get_kernel(h)))) <= REF_TOL
+    # bit-identical to the drop-in call for the same transform
+    assert np.array_equal(y[5], gpu_forward(f, x[5]))
+
+
+@pytest.mark.parametrize("L", [2, 5, 16, 120, 2048, 4096])
+def test_conv_adjoints_fp64(fs, L):
+    rng = np.random.default_rng(3 * L)
+    f = fs.create_fft(FO, L)
+    N = 2 * L
+    x = rng.standard_normal(N).astype(np.float32)
+    h = (rng.standard_normal(N) * np.exp(-np.arange(N) / max(1.0, L / 8))).astype(np.float32)
+    g = rng.standard_normal(N).astype(np.float32)
+    gx_ref, gh_ref = fp64.fo_conv_grads64(g, x, h)
+    K = gpu_get_kernel(f, h)
+    gx = gpu_batch(f, fftset_amd.OP_CONV_GRAD_INPUT, g[None], K[None])[0]
+    gh = gpu_batch(f, fftset_amd.OP_CONV_GRAD_KERNEL, x[None], g[None])[0]
+    assert fp64.rel_rms(gx, gx_ref) <= fp64_tol(L)
+    assert fp64.rel_rms(gh, gh_ref) <= fp64_tol(L)
+
+
+def test_unsupported_sizes_return_null(fs):
+    for n in (0, 1, 7, 25, 50, 11 * 16):
+        with pytest.raises(fftset_amd.FFTSetError) as e:
+            fs.create_fft(FO, n)
+        assert e.value.code == fftset_amd.E_UNSUPPORTED
+
+
+def test_plan_cache_returns_same_plan(fs):
+    lib = fftset_amd.load_library()
+    a = lib.fftset_create_fft(fs.address, fftset_amd._modulation(FO), 96)
+    b = lib.fftset_create_fft(fs.address, fftset_amd._modulation(FO), 96)
+    c = lib.fftset_create_fft(fs.address, fftset_amd._modulation(CX), 96)
+    assert a and a == b and c != a
