@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -134,12 +135,154 @@ static int run_lds(const fftset_fft* p, int op, long long count, float* out, lon
     return launch_lds(p->mod, op, a, sh, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
 }
 
+// ------------------------------------------------------------- four-step
+static bool col_shape(const ChainD& c, ShapeD* out)
+{
+    ShapeD one;
+    if (!choose_shape(c, 16, 1, &one))
+        return false;
+    int C = 16;
+    while (C > 1 && (C * one.T > 1024 || (long long)C * c.n * 8 > kMaxLdsBytes)) C /= 2;
+    if (C * one.T > 1024 || (long long)C * c.n * 8 > kMaxLdsBytes)
+        return false;
+    out->E = 16;
+    out->T = one.T;
+    out->G = C;
+    out->threads = ((C * one.T + 63) / 64) * 64;
+    out->lds_bytes = C * c.n * 8;
+    return true;
+}
+
+static bool row_shape(const ChainD& c, int rows_avail, ShapeD* out)
+{
+    if (!choose_shape(c, 16, rows_avail, out))
+        return false;
+    return out->E == 16;
+}
+
+static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
+{
+    const unsigned L = p->lfft;
+    const unsigned tab_n = p->mod == MOD_FO ? 4 * L : L;
+    const Table* tab = get_table(ctx, tab_n, st);
+    if (!tab)
+        return FFTSET_E_NOMEM;
+    // the most balanced split L = A * B with A <= B that both kernels accept
+    FourStep best{};
+    double best_score = 1e30;
+    for (unsigned A = 2; A * A <= L; ++A) {
+        if (L % A)
+            continue;
+        const unsigned B = L / A;
+        FourStep fs{};
+        fs.A = A;
+        fs.B = B;
+        if (!fits_lds(A) || !fits_lds(B))
+            continue;
+        if (!build_chain(default_chain(A), A, tab->M, &fs.colc) || !build_chain(default_chain(B), B, tab->M, &fs.rowc))
+            continue;
+        if (!col_shape(fs.colc, &fs.cols) || !row_shape(fs.rowc, (int)A, &fs.rows))
+            continue;
+        const double score = std::abs(std::log2((double)B) - std::log2((double)A));
+        if (score < best_score) {
+            best_score = score;
+            best = fs;
+        }
+    }
+    if (best_score > 1e29)
+        return FFTSET_E_UNSUPPORTED;
+    best.tw_n_stride = (int)(tab->M / L);
+    p->fs = best;
+    p->tw = tab->dev;
+    p->M = tab->M;
+    p->ms = p->mod == MOD_FO ? (int)(tab->M / (4 * L)) : 0;
+    p->engine = ENG_FOUR_STEP;
+    return FFTSET_OK;
+}
+
+static int run_four_step(const fftset_fft* p, int op, long long count, float* out, long long out_st,
+                         const float* in, long long in_st, const float* kern, long long kern_st, float* work,
+                         long long work_st, hipStream_t st)
+{
+    const FourStep& fs = p->fs;
+    const bool fo = p->mod == MOD_FO;
+    FsArgs base{};
+    base.tw = p->tw;
+    base.ms = p->ms;
+    base.twn = fs.tw_n_stride;
+    base.L = (int)p->lfft;
+    base.A = (int)fs.A;
+    base.B = (int)fs.B;
+    base.ntrans = count;
+    base.conj_kern = op == OP_GRAD_IN;
+    FsArgs ca = base, ra = base;
+    ca.c = fs.colc;
+    ca.C = fs.cols.G;
+    ca.T = fs.cols.T;
+    ca.groups = (int)((fs.B + ca.C - 1) / ca.C);
+    ra.c = fs.rowc;
+    ra.C = fs.rows.G;
+    ra.T = fs.rows.T;
+    ra.groups = (int)((fs.A + ra.C - 1) / ra.C);
+    auto io = [](FsArgs& a, const float* i, long long ist, float* o, long long ost) {
+        a.in = i;
+        a.in_st = ist;
+        a.out = o;
+        a.out_st = ost;
+    };
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    switch (op) {
+    case OP_FWD:
+        io(ca, in, in_st, work, work_st);
+        io(ra, work, work_st, out, out_st);
+        if (!ok(launch_cols(fo ? CL_FOMOD : CL_CX, CS_WORKTW, ca, fs.cols, st)))
+            return FFTSET_E_DEVICE;
+        return ok(launch_rows(RK_STOCK, fo ? RS_FOBINS : RS_CX, ra, fs.rows, st)) ? FFTSET_OK : FFTSET_E_DEVICE;
+    case OP_INV:
+        io(ca, in, in_st, work, work_st);
+        io(ra, work, work_st, out, out_st);
+        if (!ok(launch_cols(fo ? CL_FOBINS : CL_CXCONJ, CS_WORKTW, ca, fs.cols, st)))
+            return FFTSET_E_DEVICE;
+        return ok(launch_rows(RK_STOCK, fo ? RS_FODEMOD : RS_CXCONJ, ra, fs.rows, st)) ? FFTSET_OK
+                                                                                       : FFTSET_E_DEVICE;
+    case OP_KERN:
+        io(ca, in, in_st, out, out_st);
+        io(ra, out, out_st, out, out_st);
+        if (!ok(launch_cols(fo ? CL_FOMOD : CL_CX, CS_WORKTW, ca, fs.cols, st)))
+            return FFTSET_E_DEVICE;
+        return ok(launch_rows(RK_KERN, RS_NONE, ra, fs.rows, st)) ? FFTSET_OK : FFTSET_E_DEVICE;
+    case OP_CONV:
+    case OP_GRAD_IN: {
+        io(ca, in, in_st, work, work_st);
+        io(ra, work, work_st, work, work_st);
+        ra.kern = kern;
+        ra.kern_st = kern_st;
+        FsArgs cb = ca;
+        io(cb, work, work_st, out, out_st);
+        if (!ok(launch_cols(fo ? CL_FOMOD : CL_CX, CS_WORKTW, ca, fs.cols, st)))
+            return FFTSET_E_DEVICE;
+        if (!ok(launch_rows(RK_CONV, RS_NONE, ra, fs.rows, st)))
+            return FFTSET_E_DEVICE;
+        return ok(launch_cols(CL_CX, fo ? CS_FODEMOD : CS_CXCONJ, cb, fs.cols, st)) ? FFTSET_OK : FFTSET_E_DEVICE;
+    }
+    }
+    return FFTSET_E_INVALID;
+}
+
 static int run_op(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
-                  long long in_st, const float* kern, long long kern_st, hipStream_t st)
+                  long long in_st, const float* kern, long long kern_st, float* work, long long work_st,
+                  hipStream_t st)
 {
     if (p->engine == ENG_LDS)
         return run_lds(p, op, count, out, out_st, in, in_st, kern, kern_st, st);
-    return FFTSET_E_UNSUPPORTED;
+    const long long len2 = 2LL * p->lfft;
+    if (op != OP_KERN && !work) {
+        int rc = ensure_scratch(p->ctx, (size_t)(count * len2), &work);
+        if (rc != FFTSET_OK)
+            return rc;
+        work_st = len2;
+    }
+    return run_four_step(p, op, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st);
 }
 
 int execute(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
@@ -161,29 +304,42 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
     if (((uintptr_t)out | (uintptr_t)in | (uintptr_t)kern) & 7)
         return FFTSET_E_INVALID;
 
+    if (work && count > 1 && work_st < len2)
+        return FFTSET_E_INVALID;
     switch (op) {
-    case FFTSET_OP_FORWARD: return run_op(p, OP_FWD, count, out, out_st, in, in_st, nullptr, 0, st);
-    case FFTSET_OP_INVERSE: return run_op(p, OP_INV, count, out, out_st, in, in_st, nullptr, 0, st);
-    case FFTSET_OP_GET_KERNEL: return run_op(p, OP_KERN, count, out, out_st, in, in_st, nullptr, 0, st);
-    case FFTSET_OP_CONV: return run_op(p, OP_CONV, count, out, out_st, in, in_st, kern, kern_st, st);
-    case FFTSET_OP_CONV_GRAD_INPUT: return run_op(p, OP_GRAD_IN, count, out, out_st, in, in_st, kern, kern_st, st);
+    case FFTSET_OP_FORWARD: return run_op(p, OP_FWD, count, out, out_st, in, in_st, nullptr, 0, work, work_st, st);
+    case FFTSET_OP_INVERSE: return run_op(p, OP_INV, count, out, out_st, in, in_st, nullptr, 0, work, work_st, st);
+    case FFTSET_OP_GET_KERNEL:
+        return run_op(p, OP_KERN, count, out, out_st, in, in_st, nullptr, 0, nullptr, 0, st);
+    case FFTSET_OP_CONV:
+        return run_op(p, OP_CONV, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st);
+    case FFTSET_OP_CONV_GRAD_INPUT:
+        return run_op(p, OP_GRAD_IN, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st);
     case FFTSET_OP_CONV_GRAD_KERNEL: {
-        // spectrum of the signal into work, then the signal-adjoint pass of
-        // the output gradient (kern) against it
+        // spectrum of the signal (in) into a scratch region w, then the
+        // signal-adjoint pass of the output gradient (kern) against it.
+        // Four-step plans need a second region for their intermediate.
+        const bool four = p->engine == ENG_FOUR_STEP;
         float* w = work;
         long long wst = work_st;
-        if (!w) {
-            int rc = ensure_scratch(p->ctx, (size_t)(count * len2), &w);
+        float* w2 = nullptr;
+        if (!w || four) {
+            const size_t need = (size_t)(count * len2) * ((!w && four) ? 2 : 1);
+            float* s = nullptr;
+            int rc = ensure_scratch(p->ctx, need, &s);
             if (rc != FFTSET_OK)
                 return rc;
-            wst = len2;
-        } else if (count > 1 && wst < len2) {
-            return FFTSET_E_INVALID;
+            if (!w) {
+                w = s;
+                wst = len2;
+                s += count * len2;
+            }
+            w2 = four ? s : nullptr;
         }
-        int rc = run_op(p, OP_KERN, count, w, wst, in, in_st, nullptr, 0, st);
+        int rc = run_op(p, OP_KERN, count, w, wst, in, in_st, nullptr, 0, nullptr, 0, st);
         if (rc != FFTSET_OK)
             return rc;
-        return run_op(p, OP_GRAD_IN, count, out, out_st, kern, kern_st, w, wst, st);
+        return run_op(p, OP_GRAD_IN, count, out, out_st, kern, kern_st, w, wst, w2, len2, st);
     }
     }
     return FFTSET_E_INVALID;
@@ -290,6 +446,8 @@ const struct fftset_fft* fftset_create_fft(struct fftset* fc, const struct fftse
     p->ctx = ctx;
     p->mod = modulation->kind;
     int rc = fits_lds(complex_bins) ? plan_lds(p, ctx, st) : FFTSET_E_UNSUPPORTED;
+    if (rc == FFTSET_E_UNSUPPORTED)
+        rc = plan_four_step(p, ctx, st);
     if (rc != FFTSET_OK) {
         delete p;
         set_error(rc);

@@ -36,11 +36,12 @@ struct TileContig {
     __device__ __forceinline__ float2& at(int e) const { return base[lds_pad(e)]; }
 };
 
-// Element e of tile g at lds[e * C + g]: used when lanes run across tiles.
+// Element e of tile g at lds[e * C + g] (base = lds + g): tiles interleaved
+// so that lanes running across tiles touch consecutive banks.
 struct TileInter {
-    float2* lds;
+    float2* base;
     int C;
-    __device__ __forceinline__ float2& at(int e) const { return lds[e * C]; }
+    __device__ __forceinline__ float2& at(int e) const { return base[e * C]; }
 };
 
 // ------------------------------------------------------------ pass functions
@@ -333,6 +334,52 @@ template <bool CONJ>
 struct StCx {
     float2* __restrict__ out;
     __device__ __forceinline__ void operator()(int f, float2 v) const { out[f] = CONJ ? cconj(v) : v; }
+};
+
+// Writes into the tile (natural order) instead of global memory.
+template <class Tile>
+struct TileStore {
+    Tile tl;
+    __device__ __forceinline__ void operator()(int f, float2 v) const { tl.at(f) = v; }
+};
+
+// ------------------------------------------------- four-step index views
+// A transform of n = A*B is viewed as B columns of length A (element n1 of
+// column n2 is element B*n1 + n2) and A rows of length B.
+template <class Ld>
+struct LdCol {
+    Ld ld;
+    int B, n2;
+    __device__ __forceinline__ float2 operator()(int n1) const { return ld(B * n1 + n2); }
+};
+
+template <class St>
+struct StCol {
+    St st;
+    int B, n2;
+    __device__ __forceinline__ void operator()(int n1, float2 v) const { st(B * n1 + n2, v); }
+};
+
+// Column-step output: Y[k1][n2] W_n^{n2 k1} into the row-major work layout.
+struct StWorkTw {
+    float2* __restrict__ work;
+    int B, n2;
+    const float2* __restrict__ tw;
+    int twn;  // W_n^x = tw[x * twn]
+    __device__ __forceinline__ void operator()(int k1, float2 v) const
+    {
+        work[k1 * B + n2] = cmul(v, tw[(n2 * k1) * twn]);
+    }
+};
+
+// Row-step output of the convolution: element n2 of row k1, times
+// W_n^{n2 k1}, back into the row.
+struct StRowTw {
+    float2* __restrict__ row;
+    int k1;
+    const float2* __restrict__ tw;
+    int twn;
+    __device__ __forceinline__ void operator()(int n2, float2 v) const { row[n2] = cmul(v, tw[(n2 * k1) * twn]); }
 };
 
 }  // namespace fsa

@@ -65,7 +65,9 @@ def test_kat_convolution(fs, L):
 
 # ---------------------------------------------------------------- random parity
 SIZES = [2, 3, 5, 6, 9, 10, 12, 15, 16, 18, 27, 45, 48, 64, 96, 120, 128, 240, 360, 512, 576, 768,
-         960, 1440, 2048, 3840, 4096, 4608, 6144, 9216, 15360, 16384]
+         960, 1440, 2048, 3840, 4096, 4608, 6144, 9216, 15360, 16384,
+         # beyond one LDS-resident kernel: four-step through HBM
+         12960, 18432, 20480, 61440, 65536]
 
 
 @pytest.mark.parametrize("L", SIZES)
@@ -107,7 +109,7 @@ def test_freq_offset_real_parity(fs, oracle, L):
 
 
 @pytest.mark.parametrize("mod", [FO, CX])
-@pytest.mark.parametrize("L", [4, 45, 512, 3840])
+@pytest.mark.parametrize("L", [4, 45, 512, 3840, 61440])
 def test_in_place_aliasing(fs, mod, L):
     # input and output may alias for forward, inverse and conv
     from tests.gpu_util import dev, empty, host
@@ -129,12 +131,13 @@ def test_in_place_aliasing(fs, mod, L):
     assert np.array_equal(host(buf), ref_c)
 
 
-@pytest.mark.parametrize("mod,L", [(CX, 48), (FO, 120), (CX, 3840), (FO, 4096), (FO, 2048)])
+@pytest.mark.parametrize("mod,L", [(CX, 48), (FO, 120), (CX, 3840), (FO, 4096), (FO, 2048), (CX, 61440),
+                                   (FO, 61440)])
 def test_uniform_batch_matches_single_calls(fs, oracle, mod, L):
     rng = np.random.default_rng(7 * L)
     f = fs.create_fft(mod, L)
     o = oracle.plan(OFO if mod == FO else OCX, L)
-    B = 37
+    B = 37 if L < 20000 else 5
     x = rng.standard_normal((B, 2 * L)).astype(np.float32)
     h = rng.standard_normal((B, 2 * L)).astype(np.float32)
     y = gpu_batch(f, fftset_amd.OP_FORWARD, x)
@@ -145,10 +148,10 @@ def test_uniform_batch_matches_single_calls(fs, oracle, mod, L):
     c = gpu_batch(f, fftset_amd.OP_CONV, x, K)
     assert np.max(fp64.rel_rms(c, o.conv(x, o.get_kernel(h)))) <= REF_TOL
     # bit-identical to the drop-in call for the same transform
-    assert np.array_equal(y[5], gpu_forward(f, x[5]))
+    assert np.array_equal(y[B - 1], gpu_forward(f, x[B - 1]))
 
 
-@pytest.mark.parametrize("L", [2, 5, 16, 120, 2048, 4096])
+@pytest.mark.parametrize("L", [2, 5, 16, 120, 2048, 4096, 61440])
 def test_conv_adjoints_fp64(fs, L):
     rng = np.random.default_rng(3 * L)
     f = fs.create_fft(FO, L)
