@@ -58,21 +58,39 @@ static int use_device(const Ctx* ctx)
     return FFTSET_OK;
 }
 
-// Finds or creates a table of M roots with n | M.
-static const Table* get_table(Ctx* ctx, unsigned n, hipStream_t st)
+// Finds or creates the table (N, a0, rows, cols).
+static const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st)
 {
     for (const Table& t : ctx->tables)
-        if (t.M % n == 0)
-            return &t;
-    Table t{n, nullptr};
-    if (hipMalloc((void**)&t.dev, sizeof(float2) * (size_t)n) != hipSuccess)
+        if (t.N == N && t.a0 == a0 && t.rows == rows && t.cols == cols)
+            return t.dev;
+    Table t{N, a0, rows, cols, nullptr};
+    const size_t bytes = sizeof(float2) * (size_t)rows * (size_t)cols;
+    if (hipMalloc((void**)&t.dev, bytes) != hipSuccess)
         return nullptr;
-    if (launch_twiddles(t.dev, (int)n, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    if (launch_twiddles(t.dev, N, a0, rows, cols, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
         (void)hipFree(t.dev);
         return nullptr;
     }
     ctx->tables.push_back(t);
-    return &ctx->tables.back();
+    ctx->table_bytes += bytes;
+    return t.dev;
+}
+
+// Attaches the per-pass tables W_{R m}^{jk} ([k-1][j] layout) to a chain.
+static bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st)
+{
+    for (int i = 0; i < c->np; ++i) {
+        PassD& p = c->p[i];
+        if (p.m <= 1) {
+            p.tw = nullptr;
+            continue;
+        }
+        p.tw = get_table(ctx, (long long)p.R * p.m, 1, p.R - 1, p.m, st);
+        if (!p.tw)
+            return false;
+    }
+    return true;
 }
 
 static int ensure_scratch(Ctx* ctx, size_t floats, float** out)
@@ -93,21 +111,29 @@ static int ensure_scratch(Ctx* ctx, size_t floats, float** out)
     return FFTSET_OK;
 }
 
+static int attach_modulation(fftset_fft* p, Ctx* ctx, hipStream_t st)
+{
+    p->tmod = nullptr;
+    if (p->mod != MOD_FO)
+        return FFTSET_OK;
+    const unsigned L = p->lfft;
+    p->tmod = get_table(ctx, 4LL * L, 1, 1, (int)L, st);
+    return p->tmod ? FFTSET_OK : FFTSET_E_NOMEM;
+}
+
 static int plan_lds(fftset_fft* p, Ctx* ctx, hipStream_t st)
 {
     const unsigned L = p->lfft;
-    const unsigned tab_n = p->mod == MOD_FO ? 4 * L : L;
-    const Table* tab = get_table(ctx, tab_n, st);
-    if (!tab)
-        return FFTSET_E_NOMEM;
-    p->tw = tab->dev;
-    p->M = tab->M;
-    p->ms = p->mod == MOD_FO ? (int)(tab->M / (4 * L)) : 0;
     std::vector<int> radices = default_chain(L);
-    if (!build_chain(radices, L, tab->M, &p->chain))
+    if (!build_chain(radices, L, &p->chain))
         return FFTSET_E_UNSUPPORTED;
     if (!choose_shape(p->chain, 16, 0, &p->shape) && !choose_shape(p->chain, 32, 0, &p->shape))
         return FFTSET_E_UNSUPPORTED;
+    if (!attach_twiddles(ctx, &p->chain, st))
+        return FFTSET_E_NOMEM;
+    int rc = attach_modulation(p, ctx, st);
+    if (rc != FFTSET_OK)
+        return rc;
     p->engine = ENG_LDS;
     return FFTSET_OK;
 }
@@ -120,8 +146,7 @@ static int run_lds(const fftset_fft* p, int op, long long count, float* out, lon
         return FFTSET_E_UNSUPPORTED;
     LdsArgs a;
     a.c = p->chain;
-    a.tw = p->tw;
-    a.ms = p->ms;
+    a.tmod = p->tmod;
     a.L = (int)p->lfft;
     a.ntrans = count;
     a.G = sh.G;
@@ -163,10 +188,6 @@ static bool row_shape(const ChainD& c, int rows_avail, ShapeD* out)
 static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
 {
     const unsigned L = p->lfft;
-    const unsigned tab_n = p->mod == MOD_FO ? 4 * L : L;
-    const Table* tab = get_table(ctx, tab_n, st);
-    if (!tab)
-        return FFTSET_E_NOMEM;
     // the most balanced split L = A * B with A <= B that both kernels accept
     FourStep best{};
     double best_score = 1e30;
@@ -179,7 +200,7 @@ static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
         fs.B = B;
         if (!fits_lds(A) || !fits_lds(B))
             continue;
-        if (!build_chain(default_chain(A), A, tab->M, &fs.colc) || !build_chain(default_chain(B), B, tab->M, &fs.rowc))
+        if (!build_chain(default_chain(A), A, &fs.colc) || !build_chain(default_chain(B), B, &fs.rowc))
             continue;
         if (!col_shape(fs.colc, &fs.cols) || !row_shape(fs.rowc, (int)A, &fs.rows))
             continue;
@@ -191,11 +212,15 @@ static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
     }
     if (best_score > 1e29)
         return FFTSET_E_UNSUPPORTED;
-    best.tw_n_stride = (int)(tab->M / L);
+    if (!attach_twiddles(ctx, &best.colc, st) || !attach_twiddles(ctx, &best.rowc, st))
+        return FFTSET_E_NOMEM;
+    best.tfs = get_table(ctx, L, 0, (int)best.A, (int)best.B, st);
+    if (!best.tfs)
+        return FFTSET_E_NOMEM;
     p->fs = best;
-    p->tw = tab->dev;
-    p->M = tab->M;
-    p->ms = p->mod == MOD_FO ? (int)(tab->M / (4 * L)) : 0;
+    int rc = attach_modulation(p, ctx, st);
+    if (rc != FFTSET_OK)
+        return rc;
     p->engine = ENG_FOUR_STEP;
     return FFTSET_OK;
 }
@@ -207,9 +232,8 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
     const FourStep& fs = p->fs;
     const bool fo = p->mod == MOD_FO;
     FsArgs base{};
-    base.tw = p->tw;
-    base.ms = p->ms;
-    base.twn = fs.tw_n_stride;
+    base.tmod = p->tmod;
+    base.tfs = fs.tfs;
     base.L = (int)p->lfft;
     base.A = (int)fs.A;
     base.B = (int)fs.B;

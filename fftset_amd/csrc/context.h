@@ -16,11 +16,15 @@ struct fftset_modulation {
 
 namespace fsa {
 
-// A table of M-th roots of unity in HBM, W_M^k = tw[k]; shared by every plan
-// whose sizes divide M (reference: per-pass tables shared by suffix reuse,
+// A twiddle table in HBM: dev[r*cols + c] = W_N^{((a0 + r) c) mod N},
+// generated in FP64 on the GPU. Tables are keyed by (N, a0, rows, cols) and
+// shared by every plan that needs the same one -- a pass of sub-length N and
+// radix R = rows+1 is the same table in every plan containing it (the
+// reference shares its per-pass tables the same way, by suffix reuse:
 // src/fftset_vec.c:1064-1120).
 struct Table {
-    unsigned M;
+    long long N;
+    int a0, rows, cols;
     float2* dev;
 };
 
@@ -31,12 +35,13 @@ struct FourStep {
     unsigned A, B;
     ChainD colc, rowc;    // chains of the column (length A) and row (length B) transforms
     ShapeD cols, rows;    // launch shapes of the column and row kernels
-    int tw_n_stride;      // W_n^x = tw[x * tw_n_stride]
+    const float2* tfs;    // tfs[k1*B + n2] = W_n^{n2 k1}
 };
 
 struct Ctx {
     int device = 0;
     std::vector<Table> tables;
+    size_t table_bytes = 0;
     fftset_fft* first = nullptr;  // plans sorted by descending size (src/fftset.c:97-129)
     std::mutex mu;                // guards scratch
     float* scratch = nullptr;     // library-owned work space for batched calls
@@ -59,9 +64,7 @@ struct fftset_fft {
     fsa::ChainD chain;  // LDS engine chain (ENG_LDS)
     fsa::ShapeD shape;  // LDS engine launch shape for large batches
     fsa::FourStep fs;   // ENG_FOUR_STEP
-    const float2* tw;
-    unsigned M;
-    int ms;  // FREQOFFSETREAL modulation stride into tw
+    const float2* tmod;  // FREQOFFSETREAL modulation W_{4L}^e, e < L (null for COMPLEX)
 };
 
 namespace fsa {

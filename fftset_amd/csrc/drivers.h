@@ -70,7 +70,7 @@ __device__ __forceinline__ void single_kern(int lt, const Load& ld, float2* out)
 
 // Stockham chain: natural order in (through ld), natural order out (st).
 template <int E, class Tile, class Load, class Store>
-__device__ __forceinline__ void run_stockham(const ChainD& c, const float2* tw, const Tile& tl, int lt, int T,
+__device__ __forceinline__ void run_stockham(const ChainD& c, const Tile& tl, int lt, int T,
                                              const Load& ld, const Store& st)
 {
     const int np = c.np;
@@ -78,10 +78,10 @@ __device__ __forceinline__ void run_stockham(const ChainD& c, const float2* tw, 
         FSA_RADIX_SWITCH(c.p[0].R, E, (single_pass<RR>(lt, ld, st)));
         return;
     }
-    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, ld)));
+    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, c.p[0].tw, ld)));
     __syncthreads();
     for (int i = 1; i < np - 1; ++i) {
-        FSA_RADIX_SWITCH(c.p[i].R, E, (stock_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        FSA_RADIX_SWITCH(c.p[i].R, E, (stock_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
         __syncthreads();
     }
     FSA_RADIX_SWITCH(c.p[np - 1].R, E, (stock_last<RR, NB>(tl, lt, T, c.p[np - 1].s, st)));
@@ -89,7 +89,7 @@ __device__ __forceinline__ void run_stockham(const ChainD& c, const float2* tw, 
 
 // In-place DIF chain, product at the bottom, in-place DIT chain back.
 template <int E, class Tile, class Load, class Store>
-__device__ __forceinline__ void run_conv(const ChainD& c, const float2* tw, const Tile& tl, int lt, int T,
+__device__ __forceinline__ void run_conv(const ChainD& c, const Tile& tl, int lt, int T,
                                          const Load& ld, const Store& st, const BotConv& bot)
 {
     const int np = c.np;
@@ -97,24 +97,24 @@ __device__ __forceinline__ void run_conv(const ChainD& c, const float2* tw, cons
         FSA_RADIX_SWITCH(c.p[0].R, E, (single_conv<RR>(lt, ld, st, bot)));
         return;
     }
-    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, ld)));
+    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, c.p[0].tw, ld)));
     __syncthreads();
     for (int i = 1; i < np - 1; ++i) {
-        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
         __syncthreads();
     }
     FSA_RADIX_SWITCH(c.p[np - 1].R, E, (dif_bottom<RR, NB>(tl, lt, T, c.p[np - 1].s, bot)));
     __syncthreads();
     for (int i = np - 2; i >= 1; --i) {
-        FSA_RADIX_SWITCH(c.p[i].R, E, (dit_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dit_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
         __syncthreads();
     }
-    FSA_RADIX_SWITCH(c.p[0].R, E, (dit_last<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, st)));
+    FSA_RADIX_SWITCH(c.p[0].R, E, (dit_last<RR, NB>(tl, lt, T, c.p[0].m, c.p[0].tw, st)));
 }
 
 // In-place DIF chain whose bottom writes the kernel spectrum.
 template <int E, class Tile, class Load>
-__device__ __forceinline__ void run_kern(const ChainD& c, const float2* tw, const Tile& tl, int lt, int T,
+__device__ __forceinline__ void run_kern(const ChainD& c, const Tile& tl, int lt, int T,
                                          const Load& ld, float2* out)
 {
     const int np = c.np;
@@ -122,10 +122,10 @@ __device__ __forceinline__ void run_kern(const ChainD& c, const float2* tw, cons
         FSA_RADIX_SWITCH(c.p[0].R, E, (single_kern<RR>(lt, ld, out)));
         return;
     }
-    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, ld)));
+    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, c.p[0].tw, ld)));
     __syncthreads();
     for (int i = 1; i < np - 1; ++i) {
-        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
         __syncthreads();
     }
     const BotKernOut bot{out};
@@ -135,7 +135,7 @@ __device__ __forceinline__ void run_kern(const ChainD& c, const float2* tw, cons
 // Stockham chain whose last pass leaves the result in the tile, natural
 // order (for outputs that must be re-mapped before they go to HBM).
 template <int E, class Tile, class Load>
-__device__ __forceinline__ void run_stockham_to_tile(const ChainD& c, const float2* tw, const Tile& tl, int lt, int T,
+__device__ __forceinline__ void run_stockham_to_tile(const ChainD& c, const Tile& tl, int lt, int T,
                                                      const Load& ld)
 {
     const int np = c.np;
@@ -143,12 +143,12 @@ __device__ __forceinline__ void run_stockham_to_tile(const ChainD& c, const floa
         FSA_RADIX_SWITCH(c.p[0].R, E, (single_pass<RR>(lt, ld, TileStore<Tile>{tl})));
         return;
     }
-    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, tw, c.p[0].twst, ld)));
+    FSA_RADIX_SWITCH(c.p[0].R, E, (pass_first<RR, NB>(tl, lt, T, c.p[0].m, c.p[0].tw, ld)));
     __syncthreads();
     for (int i = 1; i < np; ++i) {
         if (i > 1)
             __syncthreads();
-        FSA_RADIX_SWITCH(c.p[i].R, E, (stock_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, tw, c.p[i].twst)));
+        FSA_RADIX_SWITCH(c.p[i].R, E, (stock_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
     }
 }
 

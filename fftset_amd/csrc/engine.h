@@ -17,8 +17,10 @@
 // pass writes global memory through a Store functor, so modulation, bin
 // interleave and conjugation are fused into the passes that touch HBM.
 //
-// Twiddles W_N^x come from a shared FP32 table of M-th roots of unity
-// (N divides M): W_N^x = tw[x * (M / N)].
+// Twiddles: each pass of sub-length N = R*m reads its own FP32 table laid
+// out tw[(k-1)*m + j] = W_N^{jk}, so lanes with consecutive j read
+// consecutive addresses. Tables are generated in FP64 on the GPU and shared
+// between plans by key.
 #pragma once
 
 #include "dev_math.h"
@@ -52,7 +54,7 @@ struct TileInter {
 // tile at j + k m. Shared by Stockham and in-place DIF.
 template <int R, int NB, class Tile, class Load>
 __device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m, const float2* __restrict__ tw,
-                                           int twst, const Load& ld)
+                                           const Load& ld)
 {
     float2 v[NB][R];
 #pragma unroll
@@ -69,7 +71,7 @@ __device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m,
         if (u < m) {
             DFT<R>::run(v[q]);
 #pragma unroll
-            for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[u * k * twst]);
+            for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[(k - 1) * m + u]);
 #pragma unroll
             for (int k = 0; k < R; ++k) tl.at(u + k * m) = v[q][k];
         }
@@ -78,8 +80,7 @@ __device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m,
 
 // Middle Stockham pass, LDS to LDS through registers.
 template <int R, int NB, class Tile>
-__device__ __forceinline__ void stock_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw,
-                                          int twst)
+__device__ __forceinline__ void stock_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw)
 {
     float2 v[NB][R];
     int bq[NB], jq[NB];
@@ -95,8 +96,10 @@ __device__ __forceinline__ void stock_mid(const Tile& tl, int lt, int T, int m, 
 #pragma unroll
             for (int t = 0; t < R; ++t) v[q][t] = tl.at(base + t * m);
             DFT<R>::run(v[q]);
+            if (m > 1) {
 #pragma unroll
-            for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[j * k * twst]);
+                for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[(k - 1) * m + j]);
+            }
         }
     }
     __syncthreads();
@@ -146,8 +149,7 @@ __device__ __forceinline__ void single_pass(int lt, const Load& ld, const Store&
 
 // Middle in-place DIF pass: positions b R m + j + q m, twiddle after the DFT.
 template <int R, int NB, class Tile>
-__device__ __forceinline__ void dif_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw,
-                                        int twst)
+__device__ __forceinline__ void dif_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw)
 {
     const int nbf = m * s;
 #pragma unroll
@@ -161,7 +163,7 @@ __device__ __forceinline__ void dif_mid(const Tile& tl, int lt, int T, int m, in
             for (int t = 0; t < R; ++t) v[t] = tl.at(base + t * m);
             DFT<R>::run(v);
 #pragma unroll
-            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], tw[j * k * twst]);
+            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], tw[(k - 1) * m + j]);
 #pragma unroll
             for (int k = 0; k < R; ++k) tl.at(base + k * m) = v[k];
         }
@@ -171,8 +173,7 @@ __device__ __forceinline__ void dif_mid(const Tile& tl, int lt, int T, int m, in
 // Middle in-place DIT pass: forward twiddle before a forward DFT (the
 // conjugate-data inverse of src/fftset_vec.c:78-89).
 template <int R, int NB, class Tile>
-__device__ __forceinline__ void dit_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw,
-                                        int twst)
+__device__ __forceinline__ void dit_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw)
 {
     const int nbf = m * s;
 #pragma unroll
@@ -184,7 +185,7 @@ __device__ __forceinline__ void dit_mid(const Tile& tl, int lt, int T, int m, in
             float2 v[R];
             v[0] = tl.at(base);
 #pragma unroll
-            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(base + t * m), tw[j * t * twst]);
+            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(base + t * m), tw[(t - 1) * m + j]);
             DFT<R>::run(v);
 #pragma unroll
             for (int k = 0; k < R; ++k) tl.at(base + k * m) = v[k];
@@ -195,7 +196,7 @@ __device__ __forceinline__ void dit_mid(const Tile& tl, int lt, int T, int m, in
 // Last DIT pass (s = 1, m = n/R): tile -> store in natural order j + k m.
 template <int R, int NB, class Tile, class Store>
 __device__ __forceinline__ void dit_last(const Tile& tl, int lt, int T, int m, const float2* __restrict__ tw,
-                                         int twst, const Store& st)
+                                         const Store& st)
 {
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
@@ -204,7 +205,7 @@ __device__ __forceinline__ void dit_last(const Tile& tl, int lt, int T, int m, c
             float2 v[R];
             v[0] = tl.at(u);
 #pragma unroll
-            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(u + t * m), tw[u * t * twst]);
+            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(u + t * m), tw[(t - 1) * m + u]);
             DFT<R>::run(v);
 #pragma unroll
             for (int k = 0; k < R; ++k) st(u + k * m, v[k]);
@@ -265,16 +266,15 @@ struct BotKernOut {
 
 // ------------------------------------------------------------ I/O functors
 // FREQOFFSETREAL first pass (src/fftset_mod_freqoffsetreal.c:917-924):
-// z[e] = (x[e] - i x[L+e]) e^{-i pi e / (2L)}, modulation from the table.
+// z[e] = (x[e] - i x[L+e]) e^{-i pi e / (2L)}; tmod[e] = W_{4L}^e.
 struct LdFoMod {
     const float* __restrict__ x;
     int L;
-    const float2* __restrict__ tw;
-    int ms;
+    const float2* __restrict__ tmod;
     __device__ __forceinline__ float2 operator()(int e) const
     {
         const float re = x[e], im = x[L + e];
-        const float2 w = tw[e * ms];
+        const float2 w = tmod[e];
         return make_float2(re * w.x + im * w.y, re * w.y - im * w.x);
     }
 };
@@ -320,11 +320,10 @@ struct StFoBins {
 struct StFoDemod {
     float* __restrict__ y;
     int L;
-    const float2* __restrict__ tw;
-    int ms;
+    const float2* __restrict__ tmod;
     __device__ __forceinline__ void operator()(int f, float2 v) const
     {
-        const float2 w = tw[f * ms];
+        const float2 w = tmod[f];
         y[f] = v.x * w.x - v.y * w.y;
         y[L + f] = v.x * w.y + v.y * w.x;
     }
@@ -360,26 +359,24 @@ struct StCol {
     __device__ __forceinline__ void operator()(int n1, float2 v) const { st(B * n1 + n2, v); }
 };
 
-// Column-step output: Y[k1][n2] W_n^{n2 k1} into the row-major work layout.
+// Column-step output: Y[k1][n2] W_n^{n2 k1} into the row-major work layout;
+// tfs[k1*B + n2] = W_n^{n2 k1} has the same layout.
 struct StWorkTw {
     float2* __restrict__ work;
     int B, n2;
-    const float2* __restrict__ tw;
-    int twn;  // W_n^x = tw[x * twn]
+    const float2* __restrict__ tfs;
     __device__ __forceinline__ void operator()(int k1, float2 v) const
     {
-        work[k1 * B + n2] = cmul(v, tw[(n2 * k1) * twn]);
+        work[k1 * B + n2] = cmul(v, tfs[k1 * B + n2]);
     }
 };
 
 // Row-step output of the convolution: element n2 of row k1, times
-// W_n^{n2 k1}, back into the row.
+// W_n^{n2 k1} (trow = tfs + k1*B), back into the row.
 struct StRowTw {
     float2* __restrict__ row;
-    int k1;
-    const float2* __restrict__ tw;
-    int twn;
-    __device__ __forceinline__ void operator()(int n2, float2 v) const { row[n2] = cmul(v, tw[(n2 * k1) * twn]); }
+    const float2* __restrict__ trow;
+    __device__ __forceinline__ void operator()(int n2, float2 v) const { row[n2] = cmul(v, trow[n2]); }
 };
 
 }  // namespace fsa

@@ -9,9 +9,8 @@ namespace fsa {
 // One launch of the single-kernel LDS engine over a uniform batch.
 struct LdsArgs {
     ChainD c;
-    const float2* tw;  // shared table of M-th roots of unity
-    int ms;            // FREQOFFSETREAL modulation stride: W_{4L}^e = tw[e * ms]
-    int L;             // complex bins (== c.n)
+    const float2* tmod;  // FREQOFFSETREAL modulation: tmod[e] = W_{4L}^e, e < L
+    int L;               // complex bins (== c.n)
     long long ntrans;  // transforms in the batch
     int G, T;          // transforms per workgroup, lanes per transform
     const float* in;
@@ -29,11 +28,10 @@ enum RowKind : int { RK_STOCK = 0, RK_CONV = 1, RK_KERN = 2 };
 enum RowStore : int { RS_FOBINS = 0, RS_CX = 1, RS_FODEMOD = 2, RS_CXCONJ = 3, RS_NONE = 4 };
 
 struct FsArgs {
-    ChainD c;          // column chain (length A) or row chain (length B)
-    const float2* tw;  // shared table of M roots
-    int ms;            // FREQOFFSETREAL modulation stride (W_{4n})
-    int twn;           // W_n^x = tw[x * twn]
-    int L, A, B;       // n = L = A*B
+    ChainD c;            // column chain (length A) or row chain (length B)
+    const float2* tmod;  // FREQOFFSETREAL modulation: tmod[e] = W_{4L}^e
+    const float2* tfs;   // tfs[k1*B + n2] = W_L^{n2 k1}
+    int L, A, B;         // n = L = A*B
     long long ntrans;
     int C;       // columns (or rows) per workgroup
     int T;       // lanes per column (or row) transform
@@ -47,6 +45,7 @@ struct FsArgs {
 
 hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);
 hipError_t launch_rows(int kind, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);
-hipError_t launch_twiddles(float2* tw, int M, hipStream_t st);
+// out[r*cols + c] = W_N^{((a0 + r) * c) mod N}, computed in FP64.
+hipError_t launch_twiddles(float2* out, long long N, int a0, int rows, int cols, hipStream_t st);
 
 }  // namespace fsa

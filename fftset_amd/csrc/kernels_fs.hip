@@ -36,14 +36,14 @@ __global__ void __launch_bounds__(1024) k_cols(const FsArgs a)
 
     auto run = [&](const auto& ld) {
         if constexpr (SK == CS_WORKTW)
-            run_stockham<E>(a.c, a.tw, tl, lt, a.T, ld, StWorkTw{(float2*)out, B, n2c, a.tw, a.twn});
+            run_stockham<E>(a.c, tl, lt, a.T, ld, StWorkTw{(float2*)out, B, n2c, a.tfs});
         else if constexpr (SK == CS_FODEMOD)
-            run_stockham<E>(a.c, a.tw, tl, lt, a.T, ld, StCol<StFoDemod>{{out, L, a.tw, a.ms}, B, n2c});
+            run_stockham<E>(a.c, tl, lt, a.T, ld, StCol<StFoDemod>{{out, L, a.tmod}, B, n2c});
         else
-            run_stockham<E>(a.c, a.tw, tl, lt, a.T, ld, StCol<StCx<true>>{{(float2*)out}, B, n2c});
+            run_stockham<E>(a.c, tl, lt, a.T, ld, StCol<StCx<true>>{{(float2*)out}, B, n2c});
     };
     if constexpr (LK == CL_FOMOD)
-        run(LdCol<LdFoMod>{{in, L, a.tw, a.ms}, B, n2c});
+        run(LdCol<LdFoMod>{{in, L, a.tmod}, B, n2c});
     else if constexpr (LK == CL_FOBINS)
         run(LdCol<LdFoBins>{{(const float2*)in, L}, B, n2c});
     else if constexpr (LK == CL_CX)
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
     float2* row_out = (float2*)(a.out + t0 * a.out_st) + (long long)k1c * a.B;
 
     if constexpr (KIND == RK_STOCK) {
-        run_stockham_to_tile<E>(a.c, a.tw, tl, lt, T, LdCx<false>{row_in});
+        run_stockham_to_tile<E>(a.c, tl, lt, T, LdCx<false>{row_in});
         __syncthreads();
         // transposed store: lanes run across the rows of this workgroup so
         // consecutive lanes write consecutive outputs f = k1 + A*k2
@@ -90,15 +90,15 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
             else if constexpr (SK == RS_CX)
                 StCx<false>{(float2*)out}(f, v);
             else if constexpr (SK == RS_FODEMOD)
-                StFoDemod{out, L, a.tw, a.ms}(f, v);
+                StFoDemod{out, L, a.tmod}(f, v);
             else
                 StCx<true>{(float2*)out}(f, v);
         }
     } else if constexpr (KIND == RK_CONV) {
         const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st) + (long long)k1c * a.B, a.conj_kern != 0};
-        run_conv<E>(a.c, a.tw, tl, lt, T, LdCx<false>{row_in}, StRowTw{row_out, k1c, a.tw, a.twn}, bot);
+        run_conv<E>(a.c, tl, lt, T, LdCx<false>{row_in}, StRowTw{row_out, a.tfs + (long long)k1c * a.B}, bot);
     } else {
-        run_kern<E>(a.c, a.tw, tl, lt, T, LdCx<false>{row_in}, row_out);
+        run_kern<E>(a.c, tl, lt, T, LdCx<false>{row_in}, row_out);
     }
 }
 

@@ -35,56 +35,59 @@ __global__ void __launch_bounds__(lds_max_threads<E>()) k_lds(const LdsArgs a)
 
     if constexpr (OP == OP_FWD) {
         if constexpr (MOD == MOD_FO)
-            run_stockham<E>(a.c, a.tw, tl, lt, T, LdFoMod{in, L, a.tw, a.ms}, StFoBins{(float2*)out, L});
+            run_stockham<E>(a.c, tl, lt, T, LdFoMod{in, L, a.tmod}, StFoBins{(float2*)out, L});
         else
-            run_stockham<E>(a.c, a.tw, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<false>{(float2*)out});
+            run_stockham<E>(a.c, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<false>{(float2*)out});
     } else if constexpr (OP == OP_INV) {
         if constexpr (MOD == MOD_FO)
-            run_stockham<E>(a.c, a.tw, tl, lt, T, LdFoBins{(const float2*)in, L}, StFoDemod{out, L, a.tw, a.ms});
+            run_stockham<E>(a.c, tl, lt, T, LdFoBins{(const float2*)in, L}, StFoDemod{out, L, a.tmod});
         else
-            run_stockham<E>(a.c, a.tw, tl, lt, T, LdCx<true>{(const float2*)in}, StCx<true>{(float2*)out});
+            run_stockham<E>(a.c, tl, lt, T, LdCx<true>{(const float2*)in}, StCx<true>{(float2*)out});
     } else if constexpr (OP == OP_KERN) {
         if constexpr (MOD == MOD_FO)
-            run_kern<E>(a.c, a.tw, tl, lt, T, LdFoMod{in, L, a.tw, a.ms}, (float2*)out);
+            run_kern<E>(a.c, tl, lt, T, LdFoMod{in, L, a.tmod}, (float2*)out);
         else
-            run_kern<E>(a.c, a.tw, tl, lt, T, LdCx<false>{(const float2*)in}, (float2*)out);
+            run_kern<E>(a.c, tl, lt, T, LdCx<false>{(const float2*)in}, (float2*)out);
     } else {
         const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN};
         if constexpr (MOD == MOD_FO)
-            run_conv<E>(a.c, a.tw, tl, lt, T, LdFoMod{in, L, a.tw, a.ms}, StFoDemod{out, L, a.tw, a.ms}, bot);
+            run_conv<E>(a.c, tl, lt, T, LdFoMod{in, L, a.tmod}, StFoDemod{out, L, a.tmod}, bot);
         else
-            run_conv<E>(a.c, a.tw, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<true>{(float2*)out}, bot);
+            run_conv<E>(a.c, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<true>{(float2*)out}, bot);
     }
 }
 
 // ---------------------------------------------------------------- twiddles
-// tw[k] = exp(-2 pi i k / M), evaluated in FP64 and rounded once to FP32.
-// This is the only FP64 arithmetic in the library.
-__global__ void k_twiddle_fp64(float2* __restrict__ tw, int M)
+// out[r*cols + c] = W_N^{x}, x = ((a0 + r) * c) mod N, W_N = exp(-2 pi i / N):
+// evaluated in FP64 (exact integer reduction of the angle) and rounded once
+// to FP32. This is the only FP64 arithmetic in the library.
+__global__ void k_twiddle_fp64(float2* __restrict__ out, long long N, int a0, int rows, int cols)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= M)
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)rows * cols)
         return;
-    const long long k4 = 4LL * k;
-    if (k4 % M == 0) {
-        const int q = (int)(k4 / M);  // quarter turns are exact
-        const float c[4] = {1.f, 0.f, -1.f, 0.f};
-        const float s[4] = {0.f, -1.f, 0.f, 1.f};
-        tw[k] = make_float2(c[q], s[q]);
+    const long long r = idx / cols, c = idx - r * cols;
+    long long x = ((a0 + r) % N) * (c % N) % N;
+    if ((4 * x) % N == 0) {
+        const int q = (int)((4 * x) / N);  // quarter turns are exact
+        const float cr[4] = {1.f, 0.f, -1.f, 0.f};
+        const float ci[4] = {0.f, -1.f, 0.f, 1.f};
+        out[idx] = make_float2(cr[q], ci[q]);
         return;
     }
-    long long r = k;
-    if (2 * r > M)
-        r -= M;  // angle in (-pi, pi]
-    double s, c;
-    sincospi(-2.0 * (double)r / (double)M, &s, &c);
-    tw[k] = make_float2((float)c, (float)s);
+    if (2 * x > N)
+        x -= N;  // angle in (-pi, pi]
+    double sn, cs;
+    sincospi(-2.0 * (double)x / (double)N, &sn, &cs);
+    out[idx] = make_float2((float)cs, (float)sn);
 }
 
-hipError_t launch_twiddles(float2* tw, int M, hipStream_t st)
+hipError_t launch_twiddles(float2* out, long long N, int a0, int rows, int cols, hipStream_t st)
 {
+    const long long n = (long long)rows * cols;
     const int threads = 256;
-    hipLaunchKernelGGL(k_twiddle_fp64, dim3((M + threads - 1) / threads), dim3(threads), 0, st, tw, M);
+    hipLaunchKernelGGL(k_twiddle_fp64, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads), 0, st, out, N,
+                       a0, rows, cols);
     return hipGetLastError();
 }
 

@@ -15,10 +15,11 @@ enum Mod : int { MOD_FO = 0, MOD_CX = 1 };
 enum Op : int { OP_FWD = 0, OP_INV = 1, OP_KERN = 2, OP_CONV = 3, OP_GRAD_IN = 4, OP_COUNT = 5 };
 
 struct PassD {
-    int R;     // radix of this pass
-    int m;     // sub-length after the pass (n / (s R))
-    int s;     // number of sub-transforms entering the pass
-    int twst;  // table stride: W_{R m}^x = tw[x * twst]
+    int R;  // radix of this pass
+    int m;  // sub-length after the pass (n / (s R))
+    int s;  // number of sub-transforms entering the pass
+    int pad_;
+    const float2* tw;  // tw[(k-1)*m + j] = W_{R m}^{jk}; null when m == 1
 };
 
 // One radix chain for a transform of length n.

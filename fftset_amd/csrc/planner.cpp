@@ -60,9 +60,9 @@ std::vector<int> default_chain(unsigned n)
     return r;
 }
 
-bool build_chain(const std::vector<int>& radices, unsigned n, unsigned M, ChainD* out)
+bool build_chain(const std::vector<int>& radices, unsigned n, ChainD* out)
 {
-    if (radices.empty() || (int)radices.size() > kMaxPasses || M % n)
+    if (radices.empty() || (int)radices.size() > kMaxPasses)
         return false;
     unsigned len = n, s = 1;
     out->n = (int)n;
@@ -75,7 +75,8 @@ bool build_chain(const std::vector<int>& radices, unsigned n, unsigned M, ChainD
         p.R = (int)R;
         p.m = (int)(len / R);
         p.s = (int)s;
-        p.twst = (int)(M / len);  // sub-length of this pass is len = R*m
+        p.pad_ = 0;
+        p.tw = nullptr;
         s *= R;
         len /= R;
     }

@@ -16,9 +16,8 @@ bool factor235(unsigned n, int* a, int* b, int* c);
 // from {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2}, largest first.
 std::vector<int> default_chain(unsigned n);
 
-// Fills a ChainD for `radices` over length n with twiddle strides for a
-// table of M roots (n must divide M).
-bool build_chain(const std::vector<int>& radices, unsigned n, unsigned M, ChainD* out);
+// Fills a ChainD for `radices` over length n (twiddle pointers left null).
+bool build_chain(const std::vector<int>& radices, unsigned n, ChainD* out);
 
 // Lanes per transform / transforms per workgroup for element budget E.
 // `batch_hint` caps G for small batches. Returns false when n does not fit.
