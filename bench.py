@@ -37,9 +37,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["ragged", "c2", "c3", "c5"])
+    ap.add_argument("--workload", default="ragged", choices=["ragged", "c2", "c3", "c5"])
+    ap.add_argument("--transforms", type=int, default=20000, help="ragged: batch size")
+    ap.add_argument("--threads", type=int, default=256, choices=[256, 512], help="ragged: lanes per workgroup")
+    ap.add_argument("--no-graph", action="store_true", help="ragged: launch phases directly")
+    ap.add_argument("--baselines", default="group,transform",
+                    help="ragged: comma list of baselines to time (group, transform, none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
     return ap.parse_args()
 
@@ -164,6 +169,152 @@ def workload_c5(torch, fs, rng_np, args):
     return kernels, bytes_per, cfg, check, None
 
 
+RAGGED_SIZES = [48, 120, 360, 768, 1440, 4608, 15360, 61440]
+
+
+def workload_ragged(torch, fs, rng_np, args):
+    """20000 transforms; size uniform over RAGGED_SIZES (a log-spaced set, so
+    a uniform index is the log-uniform draw); half COMPLEX forward FFTs, half
+    FREQOFFSETREAL convolutions with their own kernels (spectra precomputed
+    outside the timed region). One graph launch per step."""
+    import numpy as np
+    import fftset_amd
+    n = args.transforms
+    rng = np.random.default_rng(4)
+    sizes = np.array(RAGGED_SIZES)[rng.integers(0, len(RAGGED_SIZES), n)]
+    is_conv = rng.random(n) < 0.5
+    off = np.concatenate([[0], np.cumsum(2 * sizes)])
+    total = int(off[-1])
+    g = torch.Generator(device="cuda").manual_seed(4)
+    x = torch.randn(total, device="cuda", generator=g)
+    y = torch.empty_like(x)
+    hk = torch.randn(total, device="cuda", generator=g)  # kernel taps, then spectra in K
+    K = torch.empty_like(x)
+    plans = {}
+    for L in RAGGED_SIZES:
+        plans[("cx", L)] = fs.create_fft(fftset_amd.COMPLEX, L)
+        plans[("fo", L)] = fs.create_fft(fftset_amd.FREQ_OFFSET_REAL, L)
+    b = fs.batch()
+    kb = fs.batch()
+    eb = x.element_size()
+    base_x, base_y, base_h, base_k = x.data_ptr(), y.data_ptr(), hk.data_ptr(), K.data_ptr()
+    for i in range(n):
+        L = int(sizes[i])
+        o = int(off[i]) * eb
+        if is_conv[i]:
+            f = plans[("fo", L)]
+            kb.add(f, fftset_amd.OP_GET_KERNEL, base_k + o, base_h + o)
+            b.add(f, fftset_amd.OP_CONV, base_y + o, base_x + o, base_k + o)
+        else:
+            b.add(plans[("cx", L)], fftset_amd.OP_FORWARD, base_y + o, base_x + o)
+    kb.commit()
+    kb.launch()
+    torch.cuda.synchronize()
+    kb.destroy()
+    flags = (fftset_amd.BATCH_NO_GRAPH if args.no_graph else 0) | \
+        (fftset_amd.BATCH_THREADS_512 if args.threads == 512 else 0)
+    b.commit(flags)
+    info = b.info()
+    kernels = {"ragged_batch": lambda: b.launch()}
+    bytes_per = {"ragged_batch": b.bytes}
+    cfg = {"workload": "ragged batch: COMPLEX forward + FREQOFFSETREAL conv, one graph launch",
+           "transforms": n, "sizes": RAGGED_SIZES, "conv_fraction": round(float(is_conv.mean()), 4),
+           "algorithmic_bytes": b.bytes, "tiles": info["tiles"], "classes": info["classes"],
+           "four_step_classes": info["four_step_classes"], "lds_bytes": info["lds_bytes"],
+           "threads": args.threads, "graph": not args.no_graph}
+
+    def check():
+        from oracle import Oracle, fp64
+        from oracle.oracle import MOD_COMPLEX, MOD_FREQ_OFFSET_REAL
+        orc = Oracle()
+        worst = 0.0
+        per_size = {}
+        for L in RAGGED_SIZES:
+            for conv in (False, True):
+                idx = np.nonzero((sizes == L) & (is_conv == conv))[0][:2]
+                for i in idx:
+                    a, e = int(off[i]), int(off[i + 1])
+                    xs = x[a:e].cpu().numpy()
+                    ys = y[a:e].cpu().numpy()
+                    if conv:
+                        o = orc.plan(MOD_FREQ_OFFSET_REAL, L)
+                        ref = o.conv(xs, o.get_kernel(hk[a:e].cpu().numpy()))
+                    else:
+                        ref = orc.plan(MOD_COMPLEX, L).forward(xs)
+                    err = float(fp64.rel_rms(ys, ref))
+                    key = f"{'conv' if conv else 'fwd'}{L}"
+                    per_size[key] = max(per_size.get(key, 0.0), err)
+                    worst = max(worst, err)
+        cfg["rel_rms_vs_oracle_by_class"] = {k: float(f"{v:.3g}") for k, v in per_size.items()}
+        return worst
+
+    def cpu_sample(seconds):
+        return cpu_baseline_ragged(x, sizes, is_conv, off, hk, seconds)
+
+    def baselines(torch, steps):
+        out = {}
+        modes = [m for m in args.baselines.split(",") if m in ("group", "transform")]
+        for m in modes:
+            mode = fftset_amd.BASELINE_PER_GROUP if m == "group" else fftset_amd.BASELINE_PER_TRANSFORM
+            reps = steps if m == "group" else max(1, min(3, steps))
+            b.launch_baseline(mode)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                b.launch_baseline(mode)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            out[f"per_{m}_launches"] = {"ms": round(ms, 4), "GB/s": round(b.bytes / ms / 1e6, 2)}
+        return out
+
+    return kernels, bytes_per, cfg, check, cpu_sample, baselines
+
+
+def cpu_baseline_ragged(x_dev, sizes, is_conv, off, h_dev, seconds):
+    """Oracle on the host cores over a bounded sample of the ragged batch:
+    transforms taken in batch order, each (size, op) group of a chunk run on
+    all threads, one transform per thread. Only the transform calls are
+    timed (host-side copies and kernel spectra are prepared outside)."""
+    import numpy as np
+    from oracle import Oracle
+    from oracle.oracle import MOD_COMPLEX, MOD_FREQ_OFFSET_REAL, OP_CONV, OP_FORWARD
+    threads = min(16, os.cpu_count() or 1)
+    orc = Oracle()
+    busy = 0.0
+    done = nbytes = 0
+    n = len(sizes)
+    chunk = 4 * threads
+    i = 0
+    while busy < seconds and i < n:
+        sel = range(i, min(n, i + chunk))
+        i += chunk
+        for L in RAGGED_SIZES:
+            for conv in (False, True):
+                idx = [j for j in sel if sizes[j] == L and is_conv[j] == conv]
+                if not idx:
+                    continue
+                xs = np.stack([x_dev[int(off[j]):int(off[j + 1])].cpu().numpy() for j in idx])
+                if conv:
+                    o = orc.plan(MOD_FREQ_OFFSET_REAL, L)
+                    hs = np.stack([h_dev[int(off[j]):int(off[j + 1])].cpu().numpy() for j in idx])
+                    ks = o.get_kernel(hs)
+                    t1 = time.perf_counter()
+                    o.run_batch(OP_CONV, xs, kern=ks, nthreads=threads)
+                    busy += time.perf_counter() - t1
+                    nbytes += 24 * L * len(idx)
+                else:
+                    o = orc.plan(MOD_COMPLEX, L)
+                    t1 = time.perf_counter()
+                    o.run_batch(OP_FORWARD, xs, nthreads=threads)
+                    busy += time.perf_counter() - t1
+                    nbytes += 16 * L * len(idx)
+                done += len(idx)
+    return {"value": round(nbytes / busy / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} transforms of the ragged batch (same inputs), {busy:.1f} s of compute"}
+
+
 def cpu_baseline_uniform(x_dev, L, mod, ops, seconds, kern_dev=None, h_dev=None):
     """Oracle (FP32 C restatement of the reference) on the host cores, one
     transform per thread, over a bounded sample of the same inputs."""
@@ -204,8 +355,10 @@ def main():
     assert torch.cuda.is_available(), "bench needs a GPU"
     fs = fftset_amd.FFTSet()
     rng = np.random.default_rng(0)
-    build = {"c2": workload_c2, "c3": workload_c3, "c5": workload_c5}[args.workload]
-    kernels, bytes_per, cfg, check, cpu_sample = build(torch, fs, rng, args)
+    build = {"ragged": workload_ragged, "c2": workload_c2, "c3": workload_c3, "c5": workload_c5}[args.workload]
+    built = build(torch, fs, rng, args)
+    kernels, bytes_per, cfg, check, cpu_sample = built[:5]
+    baselines = built[5] if len(built) > 5 else None
     timer = Timer(torch)
 
     def step():
@@ -250,6 +403,10 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": None,
     }
+    if baselines is not None:
+        out["per_size_launch_baselines"] = baselines(torch, args.steps)
+        g = out["per_size_launch_baselines"]
+        out["speedup_vs_baselines"] = {k: round(v["ms"] / step_ms, 3) for k, v in g.items()}
     if not args.no_cpu and cpu_sample is not None:
         out["cpu_baseline"] = cpu_sample(args.cpu_seconds)
     fs.destroy()

@@ -81,6 +81,23 @@ def load_library():
     lib.fftset_fft_complex_bins.restype = u
     lib.fftset_fft_radices.argtypes = [vp, ctypes.POINTER(u), u]
     lib.fftset_fft_radices.restype = u
+    lib.fftset_batch_create.argtypes = [vp]
+    lib.fftset_batch_create.restype = vp
+    lib.fftset_batch_add.argtypes = [vp, vp, ctypes.c_int, fp, fp, fp]
+    lib.fftset_batch_add.restype = ctypes.c_int
+    lib.fftset_batch_commit.argtypes = [vp, u]
+    lib.fftset_batch_commit.restype = ctypes.c_int
+    lib.fftset_batch_launch.argtypes = [vp, vp]
+    lib.fftset_batch_launch.restype = ctypes.c_int
+    lib.fftset_batch_launch_baseline.argtypes = [vp, ctypes.c_int, vp]
+    lib.fftset_batch_launch_baseline.restype = ctypes.c_int
+    lib.fftset_batch_bytes.argtypes = [vp]
+    lib.fftset_batch_bytes.restype = ctypes.c_longlong
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.fftset_batch_info.argtypes = [vp, ip, ip, ip, ip]
+    lib.fftset_batch_info.restype = ctypes.c_int
+    lib.fftset_batch_destroy.argtypes = [vp]
+    lib.fftset_batch_destroy.restype = None
     _lib = lib
     return lib
 
@@ -103,6 +120,12 @@ def _ptr(x):
             raise ValueError("buffer must be float32")
         return x.data_ptr()
     raise TypeError(f"not a buffer: {type(x)}")
+
+
+def _clear():
+    """Drops a failure recorded by an earlier call on this thread, so the
+    check after a drop-in call reports that call only."""
+    load_library().fftset_last_error()
 
 
 def _check(what):
@@ -135,27 +158,33 @@ class FFT:
 
     # drop-in calls (synchronous)
     def forward(self, out, inp, work):
+        _clear()
         load_library().fftset_fft_forward(self.handle, _ptr(out), _ptr(inp), _ptr(work))
         _check("forward")
 
     def inverse(self, out, inp, work):
+        _clear()
         load_library().fftset_fft_inverse(self.handle, _ptr(out), _ptr(inp), _ptr(work))
         _check("inverse")
 
     def conv_get_kernel(self, out, inp):
+        _clear()
         load_library().fftset_fft_conv_get_kernel(self.handle, _ptr(out), _ptr(inp))
         _check("conv_get_kernel")
 
     def conv(self, out, inp, kern, work):
+        _clear()
         load_library().fftset_fft_conv(self.handle, _ptr(out), _ptr(inp), _ptr(kern), _ptr(work))
         _check("conv")
 
     def conv_grad_input(self, grad_in, grad_out, kern, work):
+        _clear()
         load_library().fftset_fft_conv_grad_input(self.handle, _ptr(grad_in), _ptr(grad_out), _ptr(kern),
                                                   _ptr(work))
         _check("conv_grad_input")
 
     def conv_grad_kernel(self, grad_kern, grad_out, inp, work):
+        _clear()
         load_library().fftset_fft_conv_grad_kernel(self.handle, _ptr(grad_kern), _ptr(grad_out), _ptr(inp),
                                                    _ptr(work))
         _check("conv_grad_kernel")
@@ -183,6 +212,80 @@ class FFT:
             _ptr(work), work_stride, stream or None)
         if rc != OK:
             raise FFTSetError(rc, "execute")
+
+
+BATCH_NO_GRAPH = 1
+BATCH_THREADS_512 = 2
+BASELINE_PER_GROUP = 1
+BASELINE_PER_TRANSFORM = 2
+
+
+def _stream_handle(stream):
+    if stream is None:
+        try:
+            import torch
+            return torch.cuda.current_stream().cuda_stream or None
+        except Exception:
+            return None
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream or None
+    return stream or None
+
+
+class Batch:
+    """``struct fftset_batch``: a ragged batch run as one graph launch."""
+
+    def __init__(self, fs):
+        self._fs = fs
+        self._keep = []  # buffers must outlive the batch
+        self.handle = load_library().fftset_batch_create(fs.address)
+        if not self.handle:
+            raise FFTSetError(load_library().fftset_last_error(), "batch_create")
+
+    def add(self, fft, op, out, inp, kern=None):
+        self._keep.extend(b for b in (out, inp, kern) if b is not None and not isinstance(b, int))
+        rc = load_library().fftset_batch_add(self.handle, fft.handle, op, _ptr(out), _ptr(inp), _ptr(kern))
+        if rc < 0:
+            raise FFTSetError(rc, "batch_add")
+        return rc
+
+    def commit(self, flags=0):
+        rc = load_library().fftset_batch_commit(self.handle, flags)
+        if rc != OK:
+            raise FFTSetError(rc, "batch_commit")
+
+    def launch(self, stream=None):
+        rc = load_library().fftset_batch_launch(self.handle, _stream_handle(stream))
+        if rc != OK:
+            raise FFTSetError(rc, "batch_launch")
+
+    def launch_baseline(self, mode, stream=None):
+        rc = load_library().fftset_batch_launch_baseline(self.handle, mode, _stream_handle(stream))
+        if rc != OK:
+            raise FFTSetError(rc, "batch_launch_baseline")
+
+    @property
+    def bytes(self):
+        return int(load_library().fftset_batch_bytes(self.handle))
+
+    def info(self):
+        vals = [ctypes.c_int() for _ in range(4)]
+        rc = load_library().fftset_batch_info(self.handle, *[ctypes.byref(v) for v in vals])
+        if rc != OK:
+            raise FFTSetError(rc, "batch_info")
+        return dict(zip(("tiles", "classes", "lds_bytes", "four_step_classes"), (v.value for v in vals)))
+
+    def destroy(self):
+        if getattr(self, "handle", None):
+            load_library().fftset_batch_destroy(self.handle)
+            self.handle = None
+            self._keep = []
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class FFTSet:
@@ -215,6 +318,9 @@ class FFTSet:
         p = FFT(self, h, modulation, int(complex_bins))
         self._plans[key] = p
         return p
+
+    def batch(self):
+        return Batch(self)
 
     def destroy(self):
         if getattr(self, "_alive", False):

@@ -27,7 +27,7 @@ namespace fsa {
 
 static thread_local int t_last_error = FFTSET_OK;
 
-static void set_error(int code)
+void set_error(int code)
 {
     if (code != FFTSET_OK)
         t_last_error = code;
@@ -35,7 +35,7 @@ static void set_error(int code)
 
 // One non-blocking-free (default-flag) stream per thread and device. Default
 // flags keep it ordered with respect to the legacy default stream.
-static hipStream_t thread_stream(int device)
+hipStream_t thread_stream(int device)
 {
     constexpr int kMaxDev = 64;
     static thread_local hipStream_t streams[kMaxDev] = {};
@@ -48,7 +48,7 @@ static hipStream_t thread_stream(int device)
     return streams[device];
 }
 
-static int use_device(const Ctx* ctx)
+int use_device(const Ctx* ctx)
 {
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess)
@@ -59,7 +59,7 @@ static int use_device(const Ctx* ctx)
 }
 
 // Finds or creates the table (N, a0, rows, cols).
-static const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st)
+const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st)
 {
     for (const Table& t : ctx->tables)
         if (t.N == N && t.a0 == a0 && t.rows == rows && t.cols == cols)
@@ -78,7 +78,7 @@ static const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols
 }
 
 // Attaches the per-pass tables W_{R m}^{jk} ([k-1][j] layout) to a chain.
-static bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st)
+bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st)
 {
     for (int i = 0; i < c->np; ++i) {
         PassD& p = c->p[i];
@@ -161,14 +161,14 @@ static int run_lds(const fftset_fft* p, int op, long long count, float* out, lon
 }
 
 // ------------------------------------------------------------- four-step
-static bool col_shape(const ChainD& c, ShapeD* out)
+static bool col_shape(const ChainD& c, int max_threads, ShapeD* out)
 {
     ShapeD one;
     if (!choose_shape(c, 16, 1, &one))
         return false;
     int C = 16;
-    while (C > 1 && (C * one.T > 1024 || (long long)C * c.n * 8 > kMaxLdsBytes)) C /= 2;
-    if (C * one.T > 1024 || (long long)C * c.n * 8 > kMaxLdsBytes)
+    while (C > 1 && (C * one.T > max_threads || (long long)C * c.n * 8 > kMaxLdsBytes)) C /= 2;
+    if (C * one.T > max_threads || (long long)C * c.n * 8 > kMaxLdsBytes)
         return false;
     out->E = 16;
     out->T = one.T;
@@ -178,16 +178,20 @@ static bool col_shape(const ChainD& c, ShapeD* out)
     return true;
 }
 
-static bool row_shape(const ChainD& c, int rows_avail, ShapeD* out)
+static bool row_shape(const ChainD& c, int rows_avail, int max_threads, ShapeD* out)
 {
-    if (!choose_shape(c, 16, rows_avail, out))
+    if (!choose_shape(c, 16, rows_avail, out) || out->E != 16)
         return false;
-    return out->E == 16;
+    while (out->G > 1 && out->G * out->T > max_threads) {
+        out->G--;
+        out->lds_bytes = out->G * lds_len(c.n) * 8;
+        out->threads = ((out->G * out->T + 63) / 64) * 64;
+    }
+    return out->G * out->T <= max_threads;
 }
 
-static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
+int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st)
 {
-    const unsigned L = p->lfft;
     // the most balanced split L = A * B with A <= B that both kernels accept
     FourStep best{};
     double best_score = 1e30;
@@ -202,7 +206,7 @@ static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
             continue;
         if (!build_chain(default_chain(A), A, &fs.colc) || !build_chain(default_chain(B), B, &fs.rowc))
             continue;
-        if (!col_shape(fs.colc, &fs.cols) || !row_shape(fs.rowc, (int)A, &fs.rows))
+        if (!col_shape(fs.colc, max_threads, &fs.cols) || !row_shape(fs.rowc, (int)A, max_threads, &fs.rows))
             continue;
         const double score = std::abs(std::log2((double)B) - std::log2((double)A));
         if (score < best_score) {
@@ -217,8 +221,16 @@ static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
     best.tfs = get_table(ctx, L, 0, (int)best.A, (int)best.B, st);
     if (!best.tfs)
         return FFTSET_E_NOMEM;
-    p->fs = best;
-    int rc = attach_modulation(p, ctx, st);
+    *out = best;
+    return FFTSET_OK;
+}
+
+static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
+{
+    int rc = build_four_step(ctx, p->lfft, 1024, &p->fs, st);
+    if (rc != FFTSET_OK)
+        return rc;
+    rc = attach_modulation(p, ctx, st);
     if (rc != FFTSET_OK)
         return rc;
     p->engine = ENG_FOUR_STEP;

@@ -1,0 +1,567 @@
+// batch.cpp -- ragged batches: thousands of transforms of mixed sizes,
+// modulations and operations run as one graph launch.
+//
+// Commit groups the transforms into classes (plan, operation), picks for
+// each class either LDS-resident tiles (G transforms per workgroup) or a
+// four-step decomposition, builds the tile table sorted largest first, and
+// captures the launches into a hipGraph:
+//   phase 1: every LDS tile + the first column step of four-step transforms
+//   phase 2: the row steps of four-step transforms
+//   phase 3: the closing column step of four-step convolutions
+// Each phase is one launch of the ragged kernel over its tiles. The same
+// kernel launched once per (class, phase), or once per transform, gives the
+// per-size-launch baselines.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "context.h"
+#include "planner.h"
+
+namespace fsa {
+
+struct BatchJob {
+    const fftset_fft* p;
+    int op;
+    float* out;
+    const float* in;
+    const float* kern;
+};
+
+struct BatchClass {
+    const fftset_fft* p;
+    int op;
+    std::vector<int> jobs;  // indices into the batch's job list
+    RClass rc;              // device descriptor (pointers patched at upload)
+    bool four_step;
+    FourStep fs;
+    int threads;            // workgroup size of this class's tiles
+};
+
+// A contiguous run of tiles launched together with one workgroup size.
+struct TileRange {
+    int first, count;
+    int threads = 256;
+    int lds_bytes = 0;
+};
+
+}  // namespace fsa
+
+using namespace fsa;
+
+struct fftset_batch {
+    Ctx* ctx = nullptr;
+    int threads = 256;
+    bool committed = false;
+    bool use_graph = true;
+    std::vector<BatchJob> jobs;
+    std::vector<BatchClass> classes;
+    // device tables
+    void* dev_mem = nullptr;
+    RClass* d_classes = nullptr;
+    RTile* d_tiles = nullptr;        // fused order: phase 1, 2, 3, each sorted largest first
+    RTile* d_group_tiles = nullptr;  // per (class, phase), for the per-group baseline
+    RTile* d_job_tiles = nullptr;    // per transform, for the per-transform baseline
+    float* work_pool = nullptr;
+    std::vector<TileRange> phase[3];  // per phase: one range per workgroup size
+    std::vector<TileRange> group_ranges;
+    std::vector<TileRange> job_ranges;
+    int lds_bytes = 0;  // largest of any range
+    long long alg_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+};
+
+namespace fsa {
+
+static long long op_bytes(int op, long long L) { return (op == OP_CONV || op == OP_GRAD_IN ? 24 : 16) * L; }
+
+static int map_op(int api_op)
+{
+    switch (api_op) {
+    case FFTSET_OP_FORWARD: return OP_FWD;
+    case FFTSET_OP_INVERSE: return OP_INV;
+    case FFTSET_OP_GET_KERNEL: return OP_KERN;
+    case FFTSET_OP_CONV: return OP_CONV;
+    case FFTSET_OP_CONV_GRAD_INPUT: return OP_GRAD_IN;
+    default: return -1;
+    }
+}
+
+static int round_threads(int lanes, int min_threads)
+{
+    int w = min_threads;
+    while (w < lanes && w < 1024) w *= 2;
+    return lanes <= w ? w : -1;
+}
+
+// Tiles of a class use the plan's own engine, radix chain and lanes per
+// transform, so every result (and every kernel-spectrum layout) matches the
+// drop-in calls on the same plan. Only the packing (transforms per
+// workgroup) and the workgroup size are chosen here.
+static int plan_class(fftset_batch* b, BatchClass& k, hipStream_t)
+{
+    const fftset_fft* p = k.p;
+    RClass& r = k.rc;
+    std::memset(&r, 0, sizeof(r));
+    r.mod = p->mod;
+    r.op = k.op;
+    r.L = (int)p->lfft;
+    r.tmod = p->tmod;
+    k.four_step = p->engine == ENG_FOUR_STEP;
+    if (!k.four_step) {
+        if (p->shape.E != 16)
+            return FFTSET_E_UNSUPPORTED;
+        const int T = p->shape.T;
+        k.threads = round_threads(T, b->threads);
+        if (k.threads < 0)
+            return FFTSET_E_UNSUPPORTED;
+        const long long lds_one = (long long)lds_len(r.L) * 8;
+        int G = std::max(1, k.threads / T);
+        while (G > 1 && (G * lds_one > 64 * 1024 || G > (int)k.jobs.size())) --G;
+        r.T = T;
+        r.G = G;
+        r.c = p->chain;
+        return FFTSET_OK;
+    }
+    k.fs = p->fs;
+    if (k.fs.cols.E != 16 || k.fs.rows.E != 16)
+        return FFTSET_E_UNSUPPORTED;
+    r.A = (int)k.fs.A;
+    r.B = (int)k.fs.B;
+    r.c = k.fs.colc;
+    r.c2 = k.fs.rowc;
+    r.tfs = k.fs.tfs;
+    r.colT = k.fs.cols.T;
+    r.colC = k.fs.cols.G;
+    r.colGroups = (r.B + r.colC - 1) / r.colC;
+    r.rowT = k.fs.rows.T;
+    r.rowR = k.fs.rows.G;
+    r.rowGroups = (r.A + r.rowR - 1) / r.rowR;
+    k.threads = round_threads(std::max(r.colC * r.colT, r.rowR * r.rowT), b->threads);
+    return k.threads < 0 ? FFTSET_E_UNSUPPORTED : FFTSET_OK;
+}
+
+static int class_lds_bytes(const BatchClass& k)
+{
+    const RClass& r = k.rc;
+    if (!k.four_step)
+        return r.G * lds_len(r.L) * 8;
+    const int col = r.colC * r.A * 8;
+    const int row = r.rowR * lds_len(r.B) * 8;
+    return std::max(col, row);
+}
+
+struct CostTile {
+    RTile t;
+    long long cost;
+};
+
+static int commit(fftset_batch* b, hipStream_t st)
+{
+    // classes in first-seen order of (plan, op)
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        const BatchJob& jb = b->jobs[j];
+        BatchClass* k = nullptr;
+        for (auto& c : b->classes)
+            if (c.p == jb.p && c.op == jb.op) {
+                k = &c;
+                break;
+            }
+        if (!k) {
+            b->classes.push_back(BatchClass{});
+            k = &b->classes.back();
+            k->p = jb.p;
+            k->op = jb.op;
+        }
+        k->jobs.push_back((int)j);
+    }
+    size_t work_floats = 0;
+    for (auto& k : b->classes) {
+        int rc = plan_class(b, k, st);
+        if (rc != FFTSET_OK)
+            return rc;
+        if (class_lds_bytes(k) > kMaxLdsBytes)
+            return FFTSET_E_UNSUPPORTED;
+        if (k.four_step && k.op != OP_KERN)
+            work_floats += 2ull * k.p->lfft * k.jobs.size();
+        b->alg_bytes += op_bytes(k.op, k.p->lfft) * (long long)k.jobs.size();
+    }
+
+    // tiles: fused (by phase, largest first), per group, per transform
+    std::vector<CostTile> ph[3];
+    std::vector<RTile> group, per_job;
+    // per transform: one tile list per phase (each phase is its own launch)
+    std::vector<std::vector<RTile>> job_tiles(3 * b->jobs.size());
+    for (int ci = 0; ci < (int)b->classes.size(); ++ci) {
+        const BatchClass& k = b->classes[ci];
+        const RClass& r = k.rc;
+        const int nj = (int)k.jobs.size();
+        const long long per = op_bytes(k.op, r.L);
+        TileRange gr[3];
+        for (int q = 0; q < 3; ++q) gr[q] = {0, 0};
+        std::vector<RTile> g3[3];
+        if (!k.four_step) {
+            for (int j = 0; j < nj; j += r.G) {
+                RTile t{ci, TK_LDS, j, std::min(r.G, nj - j)};
+                ph[0].push_back({t, per * t.count});
+                g3[0].push_back(t);
+            }
+            for (int j = 0; j < nj; ++j) job_tiles[3 * k.jobs[j]].push_back(RTile{ci, TK_LDS, j, 1});
+        } else {
+            const long long col_cost = per * r.colC / std::max(1, r.B);
+            const long long row_cost = per * r.rowR / std::max(1, r.A);
+            const bool conv = k.op == OP_CONV || k.op == OP_GRAD_IN;
+            for (int j = 0; j < nj; ++j) {
+                for (int q = 0; q < r.colGroups; ++q) {
+                    RTile t{ci, TK_COL, j, q};
+                    ph[0].push_back({t, col_cost});
+                    g3[0].push_back(t);
+                    job_tiles[3 * k.jobs[j]].push_back(t);
+                }
+            }
+            for (int j = 0; j < nj; ++j) {
+                for (int q = 0; q < r.rowGroups; ++q) {
+                    RTile t{ci, TK_ROW, j, q};
+                    ph[1].push_back({t, row_cost});
+                    g3[1].push_back(t);
+                    job_tiles[3 * k.jobs[j] + 1].push_back(t);
+                }
+            }
+            if (conv) {
+                for (int j = 0; j < nj; ++j) {
+                    for (int q = 0; q < r.colGroups; ++q) {
+                        RTile t{ci, TK_COL2, j, q};
+                        ph[2].push_back({t, col_cost});
+                        g3[2].push_back(t);
+                        job_tiles[3 * k.jobs[j] + 2].push_back(t);
+                    }
+                }
+            }
+        }
+        for (int q = 0; q < 3; ++q) {
+            if (g3[q].empty())
+                continue;
+            b->group_ranges.push_back({(int)group.size(), (int)g3[q].size(), k.threads, class_lds_bytes(k)});
+            group.insert(group.end(), g3[q].begin(), g3[q].end());
+        }
+    }
+    std::vector<RTile> fused;
+    for (int q = 0; q < 3; ++q) {
+        std::stable_sort(ph[q].begin(), ph[q].end(),
+                         [](const CostTile& x, const CostTile& y) { return x.cost > y.cost; });
+        // one range per workgroup size, each keeping the largest-first order
+        for (int w = 256; w <= 1024; w *= 2) {
+            TileRange r{(int)fused.size(), 0, w, 0};
+            for (auto& ct : ph[q]) {
+                const BatchClass& k = b->classes[ct.t.cls];
+                if (k.threads != w)
+                    continue;
+                fused.push_back(ct.t);
+                r.count++;
+                r.lds_bytes = std::max(r.lds_bytes, class_lds_bytes(k));
+            }
+            if (r.count) {
+                b->phase[q].push_back(r);
+                b->lds_bytes = std::max(b->lds_bytes, r.lds_bytes);
+            }
+        }
+    }
+    for (size_t v = 0; v < job_tiles.size(); ++v) {
+        const auto& tl = job_tiles[v];
+        TileRange r{(int)per_job.size(), (int)tl.size(), 256, 0};
+        if (!tl.empty()) {
+            const BatchClass& k = b->classes[tl[0].cls];
+            r.threads = k.threads;
+            r.lds_bytes = class_lds_bytes(k);
+        }
+        b->job_ranges.push_back(r);
+        per_job.insert(per_job.end(), tl.begin(), tl.end());
+    }
+
+    // device memory: classes, three tile tables, per-class pointer arrays.
+    // Lay out every region first (16-byte aligned), then allocate.
+    size_t off = 0;
+    auto place = [&](size_t n) {
+        size_t o = off;
+        off += (n + 15) & ~size_t(15);
+        return o;
+    };
+    const size_t o_cls = place(sizeof(RClass) * b->classes.size());
+    const size_t o_f = place(sizeof(RTile) * fused.size());
+    const size_t o_g = place(sizeof(RTile) * group.size());
+    const size_t o_j = place(sizeof(RTile) * per_job.size());
+    std::vector<size_t> o_ptr;  // four arrays per class: in, out, kern, work
+    for (auto& k : b->classes)
+        for (int q = 0; q < 4; ++q) o_ptr.push_back(place(sizeof(void*) * k.jobs.size()));
+    const size_t bytes = off;
+    if (hipMalloc(&b->dev_mem, bytes) != hipSuccess)
+        return FFTSET_E_NOMEM;
+    if (work_floats && hipMalloc((void**)&b->work_pool, work_floats * sizeof(float)) != hipSuccess)
+        return FFTSET_E_NOMEM;
+    std::vector<char> host(bytes, 0);
+    char* dbase = (char*)b->dev_mem;
+    float* wp = b->work_pool;
+    std::vector<RClass> cls;
+    for (size_t ci = 0; ci < b->classes.size(); ++ci) {
+        const BatchClass& k = b->classes[ci];
+        RClass r = k.rc;
+        const size_t nj = k.jobs.size();
+        const size_t o_in = o_ptr[4 * ci], o_out = o_ptr[4 * ci + 1];
+        const size_t o_k = o_ptr[4 * ci + 2], o_w = o_ptr[4 * ci + 3];
+        const void** hin = (const void**)(host.data() + o_in);
+        void** hout = (void**)(host.data() + o_out);
+        const void** hk = (const void**)(host.data() + o_k);
+        void** hw = (void**)(host.data() + o_w);
+        for (size_t j = 0; j < nj; ++j) {
+            const BatchJob& jb = b->jobs[k.jobs[j]];
+            hin[j] = jb.in;
+            hout[j] = jb.out;
+            hk[j] = jb.kern;
+            hw[j] = nullptr;
+            if (k.four_step && k.op != OP_KERN) {
+                hw[j] = wp;
+                wp += 2ull * k.p->lfft;
+            }
+        }
+        r.in = (const float* const*)(dbase + o_in);
+        r.out = (float* const*)(dbase + o_out);
+        r.kern = (const float* const*)(dbase + o_k);
+        r.work = (float* const*)(dbase + o_w);
+        cls.push_back(r);
+    }
+    std::memcpy(host.data() + o_cls, cls.data(), sizeof(RClass) * cls.size());
+    std::memcpy(host.data() + o_f, fused.data(), sizeof(RTile) * fused.size());
+    std::memcpy(host.data() + o_g, group.data(), sizeof(RTile) * group.size());
+    std::memcpy(host.data() + o_j, per_job.data(), sizeof(RTile) * per_job.size());
+    if (hipMemcpyAsync(b->dev_mem, host.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return FFTSET_E_DEVICE;
+    b->d_classes = (RClass*)(dbase + o_cls);
+    b->d_tiles = (RTile*)(dbase + o_f);
+    b->d_group_tiles = (RTile*)(dbase + o_g);
+    b->d_job_tiles = (RTile*)(dbase + o_j);
+    return FFTSET_OK;
+}
+
+static int launch_range(const fftset_batch* b, const RTile* tiles, TileRange r, hipStream_t st)
+{
+    if (r.count <= 0)
+        return FFTSET_OK;
+    RaggedArgs a{b->d_classes, tiles + r.first, r.count};
+    return launch_ragged(a, r.threads, r.lds_bytes, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
+}
+
+// Phases run in order; within a phase the ranges of different workgroup
+// sizes are independent and, when side streams are given, run concurrently
+// (fork / join through events, which graph capture turns into branches).
+static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
+{
+    for (int q = 0; q < 3; ++q) {
+        const auto& rs = b->phase[q];
+        if (rs.empty())
+            continue;
+        const int extra = branches ? std::min<int>((int)rs.size() - 1, 2) : 0;
+        if (extra > 0 && hipEventRecord(b->fork, st) != hipSuccess)
+            return FFTSET_E_DEVICE;
+        for (int i = 0; i < (int)rs.size(); ++i) {
+            hipStream_t s = st;
+            if (i >= 1 && i <= extra) {
+                s = b->side[i - 1];
+                if (hipStreamWaitEvent(s, b->fork, 0) != hipSuccess)
+                    return FFTSET_E_DEVICE;
+            }
+            int rc = launch_range(b, b->d_tiles, rs[i], s);
+            if (rc != FFTSET_OK)
+                return rc;
+            if (i >= 1 && i <= extra &&
+                (hipEventRecord(b->join[i - 1], s) != hipSuccess || hipStreamWaitEvent(st, b->join[i - 1], 0) != hipSuccess))
+                return FFTSET_E_DEVICE;
+        }
+    }
+    return FFTSET_OK;
+}
+
+}  // namespace fsa
+
+extern "C" {
+
+struct fftset_batch* fftset_batch_create(struct fftset* fc)
+{
+    if (!fc || !fc->impl) {
+        set_error(FFTSET_E_INVALID);
+        return nullptr;
+    }
+    fftset_batch* b = new (std::nothrow) fftset_batch();
+    if (!b) {
+        set_error(FFTSET_E_NOMEM);
+        return nullptr;
+    }
+    b->ctx = (Ctx*)fc->impl;
+    return b;
+}
+
+int fftset_batch_add(struct fftset_batch* b, const struct fftset_fft* fft, int op, float* out, const float* in,
+                     const float* kern)
+{
+    if (!b || !fft || b->committed || fft->ctx != b->ctx) {
+        set_error(FFTSET_E_INVALID);
+        return FFTSET_E_INVALID;
+    }
+    const int iop = map_op(op);
+    if (iop < 0) {
+        set_error(FFTSET_E_UNSUPPORTED);
+        return FFTSET_E_UNSUPPORTED;
+    }
+    const bool needs_kern = iop == OP_CONV || iop == OP_GRAD_IN;
+    if (!out || !in || (needs_kern && !kern) || (((uintptr_t)out | (uintptr_t)in | (uintptr_t)kern) & 7) ||
+        (iop == OP_KERN && (const float*)out == in)) {
+        set_error(FFTSET_E_INVALID);
+        return FFTSET_E_INVALID;
+    }
+    b->jobs.push_back(BatchJob{fft, iop, out, in, needs_kern ? kern : nullptr});
+    return (int)b->jobs.size() - 1;
+}
+
+int fftset_batch_commit(struct fftset_batch* b, unsigned flags)
+{
+    if (!b || b->committed) {
+        set_error(FFTSET_E_INVALID);
+        return FFTSET_E_INVALID;
+    }
+    int rc = use_device(b->ctx);
+    if (rc != FFTSET_OK) {
+        set_error(rc);
+        return rc;
+    }
+    b->threads = (flags & FFTSET_BATCH_THREADS_512) ? 512 : 256;
+    b->use_graph = !(flags & FFTSET_BATCH_NO_GRAPH);
+    hipStream_t st = thread_stream(b->ctx->device);
+    rc = commit(b, st);
+    if (rc == FFTSET_OK && b->use_graph) {
+        // capture the three phase launches once; launches replay the graph
+        bool ok = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&b->side[0], hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&b->side[1], hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&b->join[0], hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&b->join[1], hipEventDisableTiming) == hipSuccess;
+        if (!ok || hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = FFTSET_E_DEVICE;
+        } else {
+            int lrc = launch_fused(b, b->cap_stream, true);
+            hipError_t e = hipStreamEndCapture(b->cap_stream, &b->graph);
+            if (lrc != FFTSET_OK || e != hipSuccess ||
+                hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+                rc = FFTSET_E_DEVICE;
+        }
+    }
+    b->committed = rc == FFTSET_OK;
+    set_error(rc);
+    return rc;
+}
+
+int fftset_batch_launch(struct fftset_batch* b, void* stream)
+{
+    if (!b || !b->committed) {
+        set_error(FFTSET_E_INVALID);
+        return FFTSET_E_INVALID;
+    }
+    int rc = use_device(b->ctx);
+    if (rc == FFTSET_OK) {
+        if (b->exec)
+            rc = hipGraphLaunch(b->exec, (hipStream_t)stream) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
+        else
+            rc = launch_fused(b, (hipStream_t)stream, false);
+    }
+    set_error(rc);
+    return rc;
+}
+
+int fftset_batch_launch_baseline(struct fftset_batch* b, int mode, void* stream)
+{
+    if (!b || !b->committed) {
+        set_error(FFTSET_E_INVALID);
+        return FFTSET_E_INVALID;
+    }
+    int rc = use_device(b->ctx);
+    if (rc != FFTSET_OK) {
+        set_error(rc);
+        return rc;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (mode == FFTSET_BASELINE_PER_GROUP) {
+        for (const TileRange& r : b->group_ranges)
+            if ((rc = launch_range(b, b->d_group_tiles, r, st)) != FFTSET_OK)
+                break;
+    } else if (mode == FFTSET_BASELINE_PER_TRANSFORM) {
+        // one launch per transform and step (job_ranges holds 3 per transform)
+        for (const TileRange& r : b->job_ranges)
+            if ((rc = launch_range(b, b->d_job_tiles, r, st)) != FFTSET_OK)
+                break;
+    } else {
+        rc = FFTSET_E_INVALID;
+    }
+    set_error(rc);
+    return rc;
+}
+
+long long fftset_batch_bytes(const struct fftset_batch* b) { return b ? b->alg_bytes : 0; }
+
+int fftset_batch_info(const struct fftset_batch* b, int* ntiles, int* nclasses, int* lds_bytes, int* four_step_classes)
+{
+    if (!b || !b->committed)
+        return FFTSET_E_INVALID;
+    if (ntiles) {
+        int n = 0;
+        for (int q = 0; q < 3; ++q)
+            for (auto& r : b->phase[q]) n += r.count;
+        *ntiles = n;
+    }
+    if (nclasses)
+        *nclasses = (int)b->classes.size();
+    if (lds_bytes)
+        *lds_bytes = b->lds_bytes;
+    if (four_step_classes) {
+        int n = 0;
+        for (auto& k : b->classes) n += k.four_step;
+        *four_step_classes = n;
+    }
+    return FFTSET_OK;
+}
+
+void fftset_batch_destroy(struct fftset_batch* b)
+{
+    if (!b)
+        return;
+    (void)use_device(b->ctx);
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream)
+        (void)hipStreamDestroy(b->cap_stream);
+    for (int i = 0; i < 2; ++i) {
+        if (b->side[i])
+            (void)hipStreamDestroy(b->side[i]);
+        if (b->join[i])
+            (void)hipEventDestroy(b->join[i]);
+    }
+    if (b->fork)
+        (void)hipEventDestroy(b->fork);
+    if (b->dev_mem || b->work_pool)
+        (void)hipDeviceSynchronize();
+    if (b->dev_mem)
+        (void)hipFree(b->dev_mem);
+    if (b->work_pool)
+        (void)hipFree(b->work_pool);
+    delete b;
+}
+
+}  // extern "C"

@@ -68,6 +68,15 @@ struct fftset_fft {
 };
 
 namespace fsa {
+// Helpers shared by the C-ABI files.
+hipStream_t thread_stream(int device);
+int use_device(const Ctx* ctx);
+void set_error(int code);
+const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st);
+bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st);
+// Four-step split of L whose column and row tiles fit max_threads lanes.
+int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st);
+
 // Runs `op` over a uniform batch on `stream` (asynchronous). Returns an
 // FFTSET_* code.
 int execute(const fftset_fft* fft, int op, long long count, float* out, long long out_st, const float* in,

@@ -132,6 +132,134 @@ __device__ __forceinline__ void run_kern(const ChainD& c, const Tile& tl, int lt
     FSA_RADIX_SWITCH(c.p[np - 1].R, E, (dif_bottom<RR, NB>(tl, lt, T, c.p[np - 1].s, bot)));
 }
 
+// ------------------------------------------------ persistent-loop stages
+// A workgroup that loops over transforms calls load_first() for the next
+// transform right after first_to_tile() of the current one has consumed the
+// prefetch registers, so HBM latency overlaps the remaining LDS passes.
+// Chains with a single pass (n == R) use the single_* stages instead.
+
+template <int E, class Load>
+__device__ __forceinline__ void load_first(const ChainD& c, float2* pf, int lt, int T, const Load& ld)
+{
+    FSA_RADIX_SWITCH(c.p[0].R, E, (first_load<RR, NB>(pf, lt, T, c.p[0].m, ld)));
+}
+
+template <int E, class Tile>
+__device__ __forceinline__ void first_to_tile(const ChainD& c, float2* pf, const Tile& tl, int lt, int T)
+{
+    FSA_RADIX_SWITCH(c.p[0].R, E, (first_compute<RR, NB>(pf, tl, lt, T, c.p[0].m, c.p[0].tw)));
+}
+
+template <int R, class Store>
+__device__ __forceinline__ void single_fft_pf(float2* pf, int lt, const Store& st)
+{
+    if (lt == 0) {
+        DFT<R>::run(pf);
+#pragma unroll
+        for (int k = 0; k < R; ++k) st(k, pf[k]);
+    }
+}
+
+template <int R, class Store>
+__device__ __forceinline__ void single_conv_pf(float2* pf, int lt, const BotConv& bot, const Store& st)
+{
+    if (lt == 0) {
+        DFT<R>::run(pf);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float2 K = bot.kern[k];
+            pf[k] = cconj(bot.conj_kern ? cmulc(pf[k], K) : cmul(pf[k], K));
+        }
+        DFT<R>::run(pf);
+#pragma unroll
+        for (int k = 0; k < R; ++k) st(k, pf[k]);
+    }
+}
+
+// n == R: the whole transform from the prefetch registers.
+template <int E, class Store>
+__device__ __forceinline__ void pf_single_stockham(const ChainD& c, float2* pf, int lt, const Store& st)
+{
+    FSA_RADIX_SWITCH(c.p[0].R, E, (single_fft_pf<RR>(pf, lt, st)));
+}
+
+template <int E, class Store>
+__device__ __forceinline__ void pf_single_conv(const ChainD& c, float2* pf, int lt, const BotConv& bot,
+                                               const Store& st)
+{
+    FSA_RADIX_SWITCH(c.p[0].R, E, (single_conv_pf<RR>(pf, lt, bot, st)));
+}
+
+template <int E>
+__device__ __forceinline__ void pf_single_kern(const ChainD& c, float2* pf, int lt, float2* out)
+{
+    FSA_RADIX_SWITCH(c.p[0].R, E, (single_fft_pf<RR>(pf, lt, StCx<false>{out})));
+}
+
+template <int E, class Tile>
+__device__ __forceinline__ void pf_single_tile(const ChainD& c, float2* pf, int lt, const Tile& tl)
+{
+    FSA_RADIX_SWITCH(c.p[0].R, E, (single_fft_pf<RR>(pf, lt, TileStore<Tile>{tl})));
+}
+
+// Remaining Stockham passes after first_to_tile().
+template <int E, class Tile, class Store>
+__device__ __forceinline__ void stockham_tail(const ChainD& c, const Tile& tl, int lt, int T, const Store& st)
+{
+    const int np = c.np;
+    __syncthreads();
+    for (int i = 1; i < np - 1; ++i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (stock_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
+        __syncthreads();
+    }
+    FSA_RADIX_SWITCH(c.p[np - 1].R, E, (stock_last<RR, NB>(tl, lt, T, c.p[np - 1].s, st)));
+}
+
+// Remaining Stockham passes, result left in the tile in natural order.
+template <int E, class Tile>
+__device__ __forceinline__ void stockham_tail_to_tile(const ChainD& c, const Tile& tl, int lt, int T)
+{
+    const int np = c.np;
+    for (int i = 1; i < np; ++i) {
+        __syncthreads();
+        FSA_RADIX_SWITCH(c.p[i].R, E, (stock_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
+    }
+}
+
+// Remaining in-place DIF passes, product at the bottom, DIT back.
+template <int E, class Tile, class Store>
+__device__ __forceinline__ void conv_tail(const ChainD& c, const Tile& tl, int lt, int T, const Store& st,
+                                          const BotConv& bot)
+{
+    const int np = c.np;
+    __syncthreads();
+    for (int i = 1; i < np - 1; ++i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
+        __syncthreads();
+    }
+    FSA_RADIX_SWITCH(c.p[np - 1].R, E, (dif_bottom<RR, NB>(tl, lt, T, c.p[np - 1].s, bot)));
+    __syncthreads();
+    for (int i = np - 2; i >= 1; --i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dit_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
+        __syncthreads();
+    }
+    FSA_RADIX_SWITCH(c.p[0].R, E, (dit_last<RR, NB>(tl, lt, T, c.p[0].m, c.p[0].tw, st)));
+}
+
+// Remaining in-place DIF passes; the bottom writes the kernel spectrum.
+template <int E, class Tile>
+__device__ __forceinline__ void kern_tail(const ChainD& c, const Tile& tl, int lt, int T, float2* out)
+{
+    const int np = c.np;
+    __syncthreads();
+    for (int i = 1; i < np - 1; ++i) {
+        FSA_RADIX_SWITCH(c.p[i].R, E, (dif_mid<RR, NB>(tl, lt, T, c.p[i].m, c.p[i].s, c.p[i].tw)));
+        __syncthreads();
+    }
+    const BotKernOut bot{out};
+    FSA_RADIX_SWITCH(c.p[np - 1].R, E, (dif_bottom<RR, NB>(tl, lt, T, c.p[np - 1].s, bot)));
+}
+
 // Stockham chain whose last pass leaves the result in the tile, natural
 // order (for outputs that must be re-mapped before they go to HBM).
 template <int E, class Tile, class Load>

@@ -78,6 +78,40 @@ __device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m,
     }
 }
 
+// The first pass split in two so a persistent workgroup can prefetch the
+// next transform's inputs: first_load() fills pf[q*R + t] with x[u + t m]
+// (u = lt + q T); first_compute() runs the butterflies from pf into the tile.
+template <int R, int NB, class Load>
+__device__ __forceinline__ void first_load(float2* pf, int lt, int T, int m, const Load& ld)
+{
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < m) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) pf[q * R + t] = ld(u + t * m);
+        }
+    }
+}
+
+template <int R, int NB, class Tile>
+__device__ __forceinline__ void first_compute(float2* pf, const Tile& tl, int lt, int T, int m,
+                                              const float2* __restrict__ tw)
+{
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int u = lt + q * T;
+        if (u < m) {
+            float2* v = pf + q * R;
+            DFT<R>::run(v);
+#pragma unroll
+            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], tw[(k - 1) * m + u]);
+#pragma unroll
+            for (int k = 0; k < R; ++k) tl.at(u + k * m) = v[k];
+        }
+    }
+}
+
 // Middle Stockham pass, LDS to LDS through registers.
 template <int R, int NB, class Tile>
 __device__ __forceinline__ void stock_mid(const Tile& tl, int lt, int T, int m, int s, const float2* __restrict__ tw)

@@ -45,6 +45,50 @@ struct FsArgs {
 
 hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);
 hipError_t launch_rows(int kind, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);
+// ------------------------------------------------------------ ragged batch
+// A ragged batch is a table of tiles over a table of classes (one class per
+// (plan, operation)); one workgroup runs one tile. Tiles are sorted by cost,
+// largest first, so the hardware's in-order dispatch balances the load.
+enum TileKind : int { TK_LDS = 0, TK_COL = 1, TK_ROW = 2, TK_COL2 = 3 };
+
+struct RClass {
+    int mod, op;  // Mod, Op
+    int L, A, B;  // A, B > 0 for four-step classes
+    int T, G;     // LDS tiles: lanes per transform, transforms per tile
+    int colT, colC, colGroups;  // four-step column tiles
+    int rowT, rowR, rowGroups;  // four-step row tiles
+    ChainD c;   // LDS chain or column chain (length A)
+    ChainD c2;  // row chain (length B)
+    const float2* tmod;
+    const float2* tfs;
+    const float* const* in;  // per-transform buffers of this class
+    float* const* out;
+    const float* const* kern;
+    float* const* work;
+};
+
+struct RTile {
+    int cls;
+    int kind;   // TileKind
+    int job;    // first transform (TK_LDS) or the transform (four-step)
+    int count;  // transforms (TK_LDS) or the column / row group
+};
+
+struct RaggedArgs {
+    const RClass* classes;
+    const RTile* tiles;
+    int ntiles;
+};
+
+// Launches tiles [0, ntiles) of `a`; every class must fit `threads` lanes.
+hipError_t launch_ragged(const RaggedArgs& a, int threads, int lds_bytes, hipStream_t st);
+
+// Grid for a persistent kernel: workgroups resident at once, capped by the
+// number of tiles.
+long long persistent_grid(const void* fn, int threads, int lds_bytes, long long tiles);
+int persistent_waves();
+bool persistent_mode();
+
 // out[r*cols + c] = W_N^{((a0 + r) * c) mod N}, computed in FP64.
 hipError_t launch_twiddles(float2* out, long long N, int a0, int rows, int cols, hipStream_t st);
 

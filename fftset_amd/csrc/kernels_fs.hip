@@ -15,101 +15,257 @@
 
 namespace fsa {
 
-template <int LK, int SK, int E>
+// Column-step functors of one (transform, column) pair.
+template <int LK, int SK>
+struct ColIo {
+    static __device__ __forceinline__ auto ld(const FsArgs& a, long long t, int n2)
+    {
+        const float* in = a.in + t * a.in_st;
+        if constexpr (LK == CL_FOMOD)
+            return LdCol<LdFoMod>{{in, a.L, a.tmod}, a.B, n2};
+        else if constexpr (LK == CL_FOBINS)
+            return LdCol<LdFoBins>{{(const float2*)in, a.L}, a.B, n2};
+        else if constexpr (LK == CL_CX)
+            return LdCol<LdCx<false>>{{(const float2*)in}, a.B, n2};
+        else
+            return LdCol<LdCx<true>>{{(const float2*)in}, a.B, n2};
+    }
+    static __device__ __forceinline__ auto st(const FsArgs& a, long long t, int n2)
+    {
+        float* out = a.out + t * a.out_st;
+        if constexpr (SK == CS_WORKTW)
+            return StWorkTw{(float2*)out, a.B, n2, a.tfs};
+        else if constexpr (SK == CS_FODEMOD)
+            return StCol<StFoDemod>{{out, a.L, a.tmod}, a.B, n2};
+        else
+            return StCol<StCx<true>>{{(float2*)out}, a.B, n2};
+    }
+};
+
+// Persistent column step: tile = (transform, group of C adjacent columns).
+template <int LK, int SK, int E, bool PERSIST>
 __global__ void __launch_bounds__(1024) k_cols(const FsArgs a)
 {
     extern __shared__ float2 smem[];
-    const int C = a.C;
+    using Io = ColIo<LK, SK>;
+    constexpr int kIdle = 1 << 28;
+    const int C = a.C, T = a.T;
     const int g = threadIdx.x % C;
     const int lt0 = threadIdx.x / C;
-    const long long t = blockIdx.x / a.groups;
-    const int grp = (int)(blockIdx.x - t * a.groups);
-    const int n2 = grp * C + g;
-    const bool valid = lt0 < a.T && n2 < a.B && t < a.ntrans;
-    const int lt = valid ? lt0 : (1 << 28);
-    const long long t0 = valid ? t : 0;
-    const int n2c = valid ? n2 : 0;
+    const bool in_col = lt0 < T;
     const TileInter tl{smem + g, C};
-    const float* in = a.in + t0 * a.in_st;
-    float* out = a.out + t0 * a.out_st;
-    const int L = a.L, B = a.B;
 
-    auto run = [&](const auto& ld) {
-        if constexpr (SK == CS_WORKTW)
-            run_stockham<E>(a.c, tl, lt, a.T, ld, StWorkTw{(float2*)out, B, n2c, a.tfs});
-        else if constexpr (SK == CS_FODEMOD)
-            run_stockham<E>(a.c, tl, lt, a.T, ld, StCol<StFoDemod>{{out, L, a.tmod}, B, n2c});
-        else
-            run_stockham<E>(a.c, tl, lt, a.T, ld, StCol<StCx<true>>{{(float2*)out}, B, n2c});
+    if constexpr (!PERSIST) {
+        const long long t = blockIdx.x / a.groups;
+        const int n2 = (int)(blockIdx.x - t * a.groups) * C + g;
+        const bool ok = in_col && n2 < a.B && t < a.ntrans;
+        run_stockham<E>(a.c, tl, ok ? lt0 : kIdle, T, Io::ld(a, ok ? t : 0, ok ? n2 : 0),
+                        Io::st(a, ok ? t : 0, ok ? n2 : 0));
+        return;
+    }
+
+    const long long ntiles = a.ntrans * a.groups;
+    const bool single = a.c.np == 1;
+    float2 pf[E];
+
+    auto lane = [&](long long tile, long long* t, int* n2) {
+        *t = tile / a.groups;
+        *n2 = (int)(tile - *t * a.groups) * C + g;
+        const bool ok = in_col && *n2 < a.B && *t < a.ntrans;
+        if (!ok) {
+            *t = 0;
+            *n2 = 0;
+        }
+        return ok ? lt0 : kIdle;
     };
-    if constexpr (LK == CL_FOMOD)
-        run(LdCol<LdFoMod>{{in, L, a.tmod}, B, n2c});
-    else if constexpr (LK == CL_FOBINS)
-        run(LdCol<LdFoBins>{{(const float2*)in, L}, B, n2c});
-    else if constexpr (LK == CL_CX)
-        run(LdCol<LdCx<false>>{{(const float2*)in}, B, n2c});
-    else
-        run(LdCol<LdCx<true>>{{(const float2*)in}, B, n2c});
+    long long tile = blockIdx.x;
+    if (tile >= ntiles)
+        return;
+    {
+        long long t;
+        int n2;
+        const int lt = lane(tile, &t, &n2);
+        load_first<E>(a.c, pf, lt, T, Io::ld(a, t, n2));
+    }
+    while (tile < ntiles) {
+        long long t, tn;
+        int n2, n2n;
+        const int lt = lane(tile, &t, &n2);
+        const long long next = tile + gridDim.x;
+        const int ltn = next < ntiles ? lane(next, &tn, &n2n) : kIdle;
+        if (single) {
+            pf_single_stockham<E>(a.c, pf, lt, Io::st(a, t, n2));
+            if (PERSIST && next < ntiles)
+                load_first<E>(a.c, pf, ltn, T, Io::ld(a, tn, n2n));
+        } else {
+            first_to_tile<E>(a.c, pf, tl, lt, T);
+            if (PERSIST && next < ntiles)
+                load_first<E>(a.c, pf, ltn, T, Io::ld(a, tn, n2n));
+            stockham_tail<E>(a.c, tl, lt, T, Io::st(a, t, n2));
+        }
+        __syncthreads();
+        tile = next;
+    }
 }
 
-template <int KIND, int SK, int E>
+// Persistent row step: tile = (transform, group of C rows).
+template <int KIND, int SK, int E, bool PERSIST>
 __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
 {
     extern __shared__ float2 smem[];
+    constexpr int kIdle = 1 << 28;
     const int T = a.T, C = a.C;
     const int g = threadIdx.x / T;
-    const long long t = blockIdx.x / a.groups;
-    const int grp = (int)(blockIdx.x - t * a.groups);
-    const int k1 = grp * C + g;
-    const bool valid = g < C && k1 < a.A && t < a.ntrans;
-    const int lt = valid ? (int)(threadIdx.x - g * T) : (1 << 28);
-    const long long t0 = t < a.ntrans ? t : 0;
-    const int k1c = valid ? k1 : 0;
+    const int lt0 = threadIdx.x - g * T;
     const int LS = lds_len(a.B);
-    const TileContig tl{smem + (valid ? g : 0) * LS};
-    const float2* row_in = (const float2*)(a.in + t0 * a.in_st) + (long long)k1c * a.B;
-    float2* row_out = (float2*)(a.out + t0 * a.out_st) + (long long)k1c * a.B;
+    const TileContig tl{smem + (g < C ? g : 0) * LS};
 
-    if constexpr (KIND == RK_STOCK) {
-        run_stockham_to_tile<E>(a.c, tl, lt, T, LdCx<false>{row_in});
-        __syncthreads();
-        // transposed store: lanes run across the rows of this workgroup so
-        // consecutive lanes write consecutive outputs f = k1 + A*k2
-        const int k1_0 = grp * C;
-        const int nrows = min(C, a.A - k1_0);
-        if (t >= a.ntrans || nrows <= 0)
-            return;
-        float* out = a.out + t * a.out_st;
-        const int L = a.L, A = a.A;
-        for (int idx = threadIdx.x; idx < nrows * a.B; idx += blockDim.x) {
-            const int gg = idx % nrows, k2 = idx / nrows;
-            const float2 v = smem[gg * LS + lds_pad(k2)];
-            const int f = k1_0 + gg + A * k2;
-            if constexpr (SK == RS_FOBINS)
-                StFoBins{(float2*)out, L}(f, v);
-            else if constexpr (SK == RS_CX)
-                StCx<false>{(float2*)out}(f, v);
-            else if constexpr (SK == RS_FODEMOD)
-                StFoDemod{out, L, a.tmod}(f, v);
-            else
-                StCx<true>{(float2*)out}(f, v);
+    if constexpr (!PERSIST) {
+        const long long t = blockIdx.x / a.groups;
+        const int k1_0 = (int)(blockIdx.x - t * a.groups) * C;
+        const int k1 = k1_0 + g;
+        const bool ok = g < C && k1 < a.A && t < a.ntrans;
+        const int lt = ok ? lt0 : kIdle;
+        const int k1c = ok ? k1 : 0;
+        const long long t0 = t < a.ntrans ? t : 0;
+        const LdCx<false> ld{(const float2*)(a.in + t0 * a.in_st) + (long long)k1c * a.B};
+        float2* row_out = (float2*)(a.out + t0 * a.out_st) + (long long)k1c * a.B;
+        if constexpr (KIND == RK_STOCK) {
+            run_stockham_to_tile<E>(a.c, tl, lt, T, ld);
+            __syncthreads();
+            const int nrows = min(C, a.A - k1_0);
+            if (t >= a.ntrans || nrows <= 0)
+                return;
+            float* out = a.out + t * a.out_st;
+            const int L = a.L, A = a.A;
+            for (int idx = threadIdx.x; idx < nrows * a.B; idx += blockDim.x) {
+                const int gg = idx % nrows, k2 = idx / nrows;
+                const float2 v = smem[gg * LS + lds_pad(k2)];
+                const int f = k1_0 + gg + A * k2;
+                if constexpr (SK == RS_FOBINS)
+                    StFoBins{(float2*)out, L}(f, v);
+                else if constexpr (SK == RS_CX)
+                    StCx<false>{(float2*)out}(f, v);
+                else if constexpr (SK == RS_FODEMOD)
+                    StFoDemod{out, L, a.tmod}(f, v);
+                else
+                    StCx<true>{(float2*)out}(f, v);
+            }
+        } else if constexpr (KIND == RK_CONV) {
+            const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st) + (long long)k1c * a.B, a.conj_kern != 0};
+            run_conv<E>(a.c, tl, lt, T, ld, StRowTw{row_out, a.tfs + (long long)k1c * a.B}, bot);
+        } else {
+            run_kern<E>(a.c, tl, lt, T, ld, row_out);
         }
-    } else if constexpr (KIND == RK_CONV) {
-        const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st) + (long long)k1c * a.B, a.conj_kern != 0};
-        run_conv<E>(a.c, tl, lt, T, LdCx<false>{row_in}, StRowTw{row_out, a.tfs + (long long)k1c * a.B}, bot);
-    } else {
-        run_kern<E>(a.c, tl, lt, T, LdCx<false>{row_in}, row_out);
+        return;
+    }
+
+    const long long ntiles = a.ntrans * a.groups;
+    const bool single = a.c.np == 1;
+    float2 pf[E];
+
+    auto lane = [&](long long tile, long long* t, int* k1) {
+        *t = tile / a.groups;
+        *k1 = (int)(tile - *t * a.groups) * C + g;
+        const bool ok = g < C && *k1 < a.A && *t < a.ntrans;
+        if (!ok) {
+            *t = 0;
+            *k1 = 0;
+        }
+        return ok ? lt0 : kIdle;
+    };
+    auto row_ld = [&](long long t, int k1) {
+        return LdCx<false>{(const float2*)(a.in + t * a.in_st) + (long long)k1 * a.B};
+    };
+    long long tile = blockIdx.x;
+    if (tile >= ntiles)
+        return;
+    {
+        long long t;
+        int k1;
+        const int lt = lane(tile, &t, &k1);
+        load_first<E>(a.c, pf, lt, T, row_ld(t, k1));
+    }
+    while (tile < ntiles) {
+        long long t, tn;
+        int k1, k1n;
+        const int lt = lane(tile, &t, &k1);
+        const long long next = tile + gridDim.x;
+        const int ltn = next < ntiles ? lane(next, &tn, &k1n) : kIdle;
+        float2* row_out = (float2*)(a.out + t * a.out_st) + (long long)k1 * a.B;
+
+        if constexpr (KIND == RK_STOCK) {
+            if (single)
+                pf_single_tile<E>(a.c, pf, lt, tl);
+            else
+                first_to_tile<E>(a.c, pf, tl, lt, T);
+            if (PERSIST && next < ntiles)
+                load_first<E>(a.c, pf, ltn, T, row_ld(tn, k1n));
+            if (!single)
+                stockham_tail_to_tile<E>(a.c, tl, lt, T);
+            __syncthreads();
+            // transposed store: lanes run across the rows of this tile so
+            // consecutive lanes write consecutive outputs f = k1 + A*k2
+            const long long tt = tile / a.groups;
+            const int k1_0 = (int)(tile - tt * a.groups) * C;
+            const int nrows = min(C, a.A - k1_0);
+            float* out = a.out + tt * a.out_st;
+            const int L = a.L, A = a.A;
+            for (int idx = threadIdx.x; idx < nrows * a.B; idx += blockDim.x) {
+                const int gg = idx % nrows, k2 = idx / nrows;
+                const float2 v = smem[gg * LS + lds_pad(k2)];
+                const int f = k1_0 + gg + A * k2;
+                if constexpr (SK == RS_FOBINS)
+                    StFoBins{(float2*)out, L}(f, v);
+                else if constexpr (SK == RS_CX)
+                    StCx<false>{(float2*)out}(f, v);
+                else if constexpr (SK == RS_FODEMOD)
+                    StFoDemod{out, L, a.tmod}(f, v);
+                else
+                    StCx<true>{(float2*)out}(f, v);
+            }
+        } else if constexpr (KIND == RK_CONV) {
+            const BotConv bot{(const float2*)(a.kern + t * a.kern_st) + (long long)k1 * a.B, a.conj_kern != 0};
+            const StRowTw st{row_out, a.tfs + (long long)k1 * a.B};
+            if (single) {
+                pf_single_conv<E>(a.c, pf, lt, bot, st);
+                if (PERSIST && next < ntiles)
+                    load_first<E>(a.c, pf, ltn, T, row_ld(tn, k1n));
+            } else {
+                first_to_tile<E>(a.c, pf, tl, lt, T);
+                if (PERSIST && next < ntiles)
+                    load_first<E>(a.c, pf, ltn, T, row_ld(tn, k1n));
+                conv_tail<E>(a.c, tl, lt, T, st, bot);
+            }
+        } else {
+            if (single) {
+                pf_single_kern<E>(a.c, pf, lt, row_out);
+                if (PERSIST && next < ntiles)
+                    load_first<E>(a.c, pf, ltn, T, row_ld(tn, k1n));
+            } else {
+                first_to_tile<E>(a.c, pf, tl, lt, T);
+                if (PERSIST && next < ntiles)
+                    load_first<E>(a.c, pf, ltn, T, row_ld(tn, k1n));
+                kern_tail<E>(a.c, tl, lt, T, row_out);
+            }
+        }
+        __syncthreads();
+        tile = next;
     }
 }
 
 template <class F>
-static hipError_t do_launch(F fn, const FsArgs& a, const ShapeD& sh, hipStream_t st)
+static hipError_t do_launch(F fn, F fn_persist, const FsArgs& a, const ShapeD& sh, hipStream_t st)
 {
-    const long long grid = a.ntrans * a.groups;
-    if (grid <= 0)
+    const long long tiles = a.ntrans * a.groups;
+    if (tiles <= 0)
         return hipSuccess;
+    const bool persist = persistent_mode();
+    if (persist)
+        fn = fn_persist;
     if (sh.lds_bytes > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
+    const long long grid = persist ? persistent_grid((const void*)fn, sh.threads, sh.lds_bytes, tiles) : tiles;
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(sh.threads), sh.lds_bytes, st, a);
     return hipGetLastError();
 }
@@ -120,16 +276,16 @@ hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, h
         return hipErrorInvalidValue;
     if (store == CS_WORKTW) {
         switch (load) {
-        case CL_FOMOD: return do_launch(k_cols<CL_FOMOD, CS_WORKTW, 16>, a, sh, st);
-        case CL_FOBINS: return do_launch(k_cols<CL_FOBINS, CS_WORKTW, 16>, a, sh, st);
-        case CL_CX: return do_launch(k_cols<CL_CX, CS_WORKTW, 16>, a, sh, st);
-        case CL_CXCONJ: return do_launch(k_cols<CL_CXCONJ, CS_WORKTW, 16>, a, sh, st);
+        case CL_FOMOD: return do_launch(k_cols<CL_FOMOD, CS_WORKTW, 16, false>, k_cols<CL_FOMOD, CS_WORKTW, 16, true>, a, sh, st);
+        case CL_FOBINS: return do_launch(k_cols<CL_FOBINS, CS_WORKTW, 16, false>, k_cols<CL_FOBINS, CS_WORKTW, 16, true>, a, sh, st);
+        case CL_CX: return do_launch(k_cols<CL_CX, CS_WORKTW, 16, false>, k_cols<CL_CX, CS_WORKTW, 16, true>, a, sh, st);
+        case CL_CXCONJ: return do_launch(k_cols<CL_CXCONJ, CS_WORKTW, 16, false>, k_cols<CL_CXCONJ, CS_WORKTW, 16, true>, a, sh, st);
         }
     } else if (load == CL_CX) {
         if (store == CS_FODEMOD)
-            return do_launch(k_cols<CL_CX, CS_FODEMOD, 16>, a, sh, st);
+            return do_launch(k_cols<CL_CX, CS_FODEMOD, 16, false>, k_cols<CL_CX, CS_FODEMOD, 16, true>, a, sh, st);
         if (store == CS_CXCONJ)
-            return do_launch(k_cols<CL_CX, CS_CXCONJ, 16>, a, sh, st);
+            return do_launch(k_cols<CL_CX, CS_CXCONJ, 16, false>, k_cols<CL_CX, CS_CXCONJ, 16, true>, a, sh, st);
     }
     return hipErrorInvalidValue;
 }
@@ -140,17 +296,17 @@ hipError_t launch_rows(int kind, int store, const FsArgs& a, const ShapeD& sh, h
         return hipErrorInvalidValue;
     if (kind == RK_STOCK) {
         switch (store) {
-        case RS_FOBINS: return do_launch(k_rows<RK_STOCK, RS_FOBINS, 16>, a, sh, st);
-        case RS_CX: return do_launch(k_rows<RK_STOCK, RS_CX, 16>, a, sh, st);
-        case RS_FODEMOD: return do_launch(k_rows<RK_STOCK, RS_FODEMOD, 16>, a, sh, st);
-        case RS_CXCONJ: return do_launch(k_rows<RK_STOCK, RS_CXCONJ, 16>, a, sh, st);
+        case RS_FOBINS: return do_launch(k_rows<RK_STOCK, RS_FOBINS, 16, false>, k_rows<RK_STOCK, RS_FOBINS, 16, true>, a, sh, st);
+        case RS_CX: return do_launch(k_rows<RK_STOCK, RS_CX, 16, false>, k_rows<RK_STOCK, RS_CX, 16, true>, a, sh, st);
+        case RS_FODEMOD: return do_launch(k_rows<RK_STOCK, RS_FODEMOD, 16, false>, k_rows<RK_STOCK, RS_FODEMOD, 16, true>, a, sh, st);
+        case RS_CXCONJ: return do_launch(k_rows<RK_STOCK, RS_CXCONJ, 16, false>, k_rows<RK_STOCK, RS_CXCONJ, 16, true>, a, sh, st);
         }
         return hipErrorInvalidValue;
     }
     if (kind == RK_CONV)
-        return do_launch(k_rows<RK_CONV, RS_NONE, 16>, a, sh, st);
+        return do_launch(k_rows<RK_CONV, RS_NONE, 16, false>, k_rows<RK_CONV, RS_NONE, 16, true>, a, sh, st);
     if (kind == RK_KERN)
-        return do_launch(k_rows<RK_KERN, RS_NONE, 16>, a, sh, st);
+        return do_launch(k_rows<RK_KERN, RS_NONE, 16, false>, k_rows<RK_KERN, RS_NONE, 16, true>, a, sh, st);
     return hipErrorInvalidValue;
 }
 

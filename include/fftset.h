@@ -150,6 +150,49 @@ void fftset_fft_conv_grad_input(const struct fftset_fft *first_pass, float *grad
 void fftset_fft_conv_grad_kernel(const struct fftset_fft *first_pass, float *grad_kernel, const float *grad_output,
                                  const float *input_buf, float *work_buf);
 
+/* ---------------------------------------------------------- ragged batches
+ * Thousands of transforms of different sizes, modulations and operations
+ * run as one graph launch. Buffers are recorded at add time and read at
+ * every launch; they must stay valid until the batch is destroyed. */
+struct fftset_batch;
+
+#define FFTSET_BATCH_NO_GRAPH      1u  /* launch the kernels directly instead of a captured hipGraph */
+#define FFTSET_BATCH_THREADS_512   2u  /* 512-lane workgroups (default 256)                          */
+
+#define FFTSET_BASELINE_PER_GROUP      1  /* one launch per (plan, operation) group and step */
+#define FFTSET_BASELINE_PER_TRANSFORM  2  /* one launch per transform and step               */
+
+/* Creates an empty batch on the set; NULL on failure. */
+struct fftset_batch *fftset_batch_create(struct fftset *fc);
+
+/* Appends one transform: op is FFTSET_OP_FORWARD, _INVERSE, _GET_KERNEL,
+ * _CONV or _CONV_GRAD_INPUT, with the buffers of the matching drop-in call
+ * (kern for the convolutions). Returns the transform's index or an error. */
+int fftset_batch_add(struct fftset_batch *b, const struct fftset_fft *fft, int op, float *out, const float *in,
+                     const float *kern);
+
+/* Plans the tiles, uploads the descriptors and (unless
+ * FFTSET_BATCH_NO_GRAPH) captures the launches into a hipGraph. */
+int fftset_batch_commit(struct fftset_batch *b, unsigned flags);
+
+/* Runs the whole batch on a hipStream_t (stream-ordered, asynchronous). */
+int fftset_batch_launch(struct fftset_batch *b, void *stream);
+
+/* Runs the same work as FFTSET_BASELINE_* separate launches (for
+ * comparison). */
+int fftset_batch_launch_baseline(struct fftset_batch *b, int mode, void *stream);
+
+/* Algorithmic bytes of one launch: 16 L per forward / inverse / get_kernel,
+ * 24 L per convolution. */
+long long fftset_batch_bytes(const struct fftset_batch *b);
+
+/* Tile count, class count, LDS bytes per workgroup and the number of
+ * classes run as four-step transforms. */
+int fftset_batch_info(const struct fftset_batch *b, int *ntiles, int *nclasses, int *lds_bytes,
+                      int *four_step_classes);
+
+void fftset_batch_destroy(struct fftset_batch *b);
+
 /* Plan introspection (for tests, benchmarks and the autotuner). */
 unsigned fftset_fft_complex_bins(const struct fftset_fft *fft);
 /* Writes at most `max` radices of the plan's LDS chain; returns the count

@@ -1,0 +1,145 @@
+"""Ragged batches on the GPU: mixed sizes, modulations and operations in one
+graph launch, checked transform by transform against the oracle; graph
+replays and the per-size-launch baselines must reproduce the fused launch."""
+import numpy as np
+import pytest
+
+try:  # import at collection: the first import on a fresh box is slow
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+import fftset_amd
+from oracle import fp64
+from oracle.oracle import MOD_COMPLEX as OCX, MOD_FREQ_OFFSET_REAL as OFO
+from tests.gpu_util import torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+FO, CX = fftset_amd.FREQ_OFFSET_REAL, fftset_amd.COMPLEX
+REF_TOL = 2e-6
+
+
+@pytest.fixture(scope="module")
+def fs():
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    s = fftset_amd.FFTSet()
+    yield s
+    s.destroy()
+
+
+def _make_jobs(rng, spec):
+    """spec: list of (mod, L, op, count). Returns job list with host data."""
+    jobs = []
+    for mod, L, op, count in spec:
+        for _ in range(count):
+            x = rng.standard_normal(2 * L).astype(np.float32)
+            h = (rng.standard_normal(2 * L) * np.exp(-np.arange(2 * L) / max(1.0, L / 4))).astype(np.float32)
+            jobs.append(dict(mod=mod, L=L, op=op, x=x, h=h))
+    order = rng.permutation(len(jobs))
+    return [jobs[i] for i in order]
+
+
+def _build(fs, jobs, flags=0):
+    """Allocates device buffers, precomputes kernel spectra, adds to a batch."""
+    b = fs.batch()
+    for jb in jobs:
+        f = fs.create_fft(jb["mod"], jb["L"])
+        jb["f"] = f
+        jb["dx"] = torch.from_numpy(jb["x"]).cuda()
+        jb["dy"] = torch.full_like(jb["dx"], float("nan"))
+        jb["dk"] = None
+        if jb["op"] in (fftset_amd.OP_CONV, fftset_amd.OP_CONV_GRAD_INPUT):
+            jb["dk"] = torch.empty_like(jb["dx"])
+            f.conv_get_kernel(jb["dk"], torch.from_numpy(jb["h"]).cuda())
+        b.add(f, jb["op"], jb["dy"], jb["dx"], jb["dk"])
+    b.commit(flags)
+    return b
+
+
+def _oracle_ref(oracle, jb):
+    o = oracle.plan(OFO if jb["mod"] == FO else OCX, jb["L"])
+    op, x = jb["op"], jb["x"]
+    if op == fftset_amd.OP_FORWARD:
+        return o.forward(x)
+    if op == fftset_amd.OP_INVERSE:
+        return o.inverse(x)
+    if op == fftset_amd.OP_CONV:
+        return o.conv(x, o.get_kernel(jb["h"]))
+    return None
+
+
+SPEC = [
+    (CX, 48, fftset_amd.OP_FORWARD, 40), (FO, 48, fftset_amd.OP_CONV, 40),
+    (CX, 120, fftset_amd.OP_FORWARD, 20), (FO, 120, fftset_amd.OP_CONV, 20),
+    (FO, 360, fftset_amd.OP_FORWARD, 9), (CX, 360, fftset_amd.OP_INVERSE, 9),
+    (CX, 768, fftset_amd.OP_FORWARD, 7), (FO, 768, fftset_amd.OP_CONV, 7),
+    (CX, 1440, fftset_amd.OP_CONV, 5), (FO, 1440, fftset_amd.OP_INVERSE, 5),
+    (CX, 4608, fftset_amd.OP_FORWARD, 4), (FO, 4608, fftset_amd.OP_CONV, 4),
+    (CX, 15360, fftset_amd.OP_FORWARD, 3), (FO, 15360, fftset_amd.OP_CONV, 3),
+    (CX, 61440, fftset_amd.OP_FORWARD, 2), (FO, 61440, fftset_amd.OP_CONV, 2),
+    (FO, 61440, fftset_amd.OP_INVERSE, 1), (CX, 61440, fftset_amd.OP_CONV, 1),
+]
+
+
+@pytest.mark.parametrize("flags", [0, fftset_amd.BATCH_NO_GRAPH, fftset_amd.BATCH_THREADS_512])
+def test_ragged_batch_matches_oracle(fs, oracle, flags):
+    rng = np.random.default_rng(11 + flags)
+    jobs = _make_jobs(rng, SPEC)
+    b = _build(fs, jobs, flags)
+    b.launch()
+    torch.cuda.synchronize()
+    info = b.info()
+    assert info["classes"] == len({(m, L, op) for m, L, op, _ in SPEC})
+    for jb in jobs:
+        ref = _oracle_ref(oracle, jb)
+        y = jb["dy"].cpu().numpy()
+        assert np.all(np.isfinite(y)), (jb["mod"], jb["L"], jb["op"])
+        assert fp64.rel_rms(y, ref) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
+    b.destroy()
+
+
+def test_graph_replays_and_baselines_reproduce_the_launch(fs):
+    rng = np.random.default_rng(5)
+    spec = [(CX, 48, fftset_amd.OP_FORWARD, 16), (FO, 768, fftset_amd.OP_CONV, 6),
+            (FO, 4608, fftset_amd.OP_CONV, 2), (CX, 61440, fftset_amd.OP_FORWARD, 1),
+            (FO, 61440, fftset_amd.OP_CONV, 1)]
+    jobs = _make_jobs(rng, spec)
+    b = _build(fs, jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    first = [jb["dy"].cpu().numpy().copy() for jb in jobs]
+    for rep in range(3):
+        for jb in jobs:  # eager work between replays
+            jb["dy"].fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.launch()
+        torch.cuda.synchronize()
+        for jb, ref in zip(jobs, first):
+            assert np.array_equal(jb["dy"].cpu().numpy(), ref), rep
+    for mode in (fftset_amd.BASELINE_PER_GROUP, fftset_amd.BASELINE_PER_TRANSFORM):
+        for jb in jobs:
+            jb["dy"].fill_(float("nan"))
+        b.launch_baseline(mode)
+        torch.cuda.synchronize()
+        for jb, ref in zip(jobs, first):
+            assert np.array_equal(jb["dy"].cpu().numpy(), ref), mode
+    b.destroy()
+
+
+def test_batch_rejects_bad_use(fs):
+    b = fs.batch()
+    f = fs.create_fft(CX, 64)
+    x = torch.zeros(128, device="cuda")
+    with pytest.raises(fftset_amd.FFTSetError):
+        b.add(f, fftset_amd.OP_CONV_GRAD_KERNEL, x, x, x)  # two-phase op: not batchable
+    with pytest.raises(fftset_amd.FFTSetError):
+        b.add(f, fftset_amd.OP_CONV, x, x, None)  # conv needs a kernel spectrum
+    b.add(f, fftset_amd.OP_FORWARD, x, x)
+    b.commit()
+    with pytest.raises(fftset_amd.FFTSetError):
+        b.commit()
+    with pytest.raises(fftset_amd.FFTSetError):
+        b.add(f, fftset_amd.OP_FORWARD, x, x)
+    b.destroy()
