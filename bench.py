@@ -41,11 +41,16 @@ def parse():
     ap.add_argument("--transforms", type=int, default=20000, help="ragged: batch size")
     ap.add_argument("--threads", type=int, default=256, choices=[256, 512], help="ragged: lanes per workgroup")
     ap.add_argument("--no-graph", action="store_true", help="ragged: launch phases directly")
+    ap.add_argument("--sched", default="branches", choices=["branches", "phased", "serial"],
+                    help="ragged: order of the tile groups inside the graph")
     ap.add_argument("--baselines", default="group,transform",
                     help="ragged: comma list of baselines to time (group, transform, none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device once (a known-size kernel for calibrating "
+                         "rocprof FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -212,7 +217,8 @@ def workload_ragged(torch, fs, rng_np, args):
     torch.cuda.synchronize()
     kb.destroy()
     flags = (fftset_amd.BATCH_NO_GRAPH if args.no_graph else 0) | \
-        (fftset_amd.BATCH_THREADS_512 if args.threads == 512 else 0)
+        (fftset_amd.BATCH_THREADS_512 if args.threads == 512 else 0) | \
+        {"branches": 0, "phased": fftset_amd.BATCH_SCHED_PHASED, "serial": fftset_amd.BATCH_SCHED_SERIAL}[args.sched]
     b.commit(flags)
     info = b.info()
     kernels = {"ragged_batch": lambda: b.launch()}
@@ -221,7 +227,7 @@ def workload_ragged(torch, fs, rng_np, args):
            "transforms": n, "sizes": RAGGED_SIZES, "conv_fraction": round(float(is_conv.mean()), 4),
            "algorithmic_bytes": b.bytes, "tiles": info["tiles"], "classes": info["classes"],
            "four_step_classes": info["four_step_classes"], "lds_bytes": info["lds_bytes"],
-           "threads": args.threads, "graph": not args.no_graph}
+           "threads": args.threads, "graph": not args.no_graph, "schedule": args.sched}
 
     def check():
         from oracle import Oracle, fp64
@@ -353,6 +359,13 @@ def main():
 
     import fftset_amd
     assert torch.cuda.is_available(), "bench needs a GPU"
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     fs = fftset_amd.FFTSet()
     rng = np.random.default_rng(0)
     build = {"ragged": workload_ragged, "c2": workload_c2, "c3": workload_c3, "c5": workload_c5}[args.workload]

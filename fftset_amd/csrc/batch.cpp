@@ -66,7 +66,9 @@ struct fftset_batch {
     RTile* d_group_tiles = nullptr;  // per (class, phase), for the per-group baseline
     RTile* d_job_tiles = nullptr;    // per transform, for the per-transform baseline
     float* work_pool = nullptr;
-    std::vector<TileRange> phase[3];  // per phase: one range per workgroup size
+    std::vector<TileRange> lds_r;    // LDS-resident tiles, one range per workgroup size
+    std::vector<TileRange> fs_r[3];  // four-step steps (columns, rows, closing columns)
+    int sched = 0;                   // FFTSET_BATCH_SCHED_*
     std::vector<TileRange> group_ranges;
     std::vector<TileRange> job_ranges;
     int lds_bytes = 0;  // largest of any range
@@ -74,8 +76,8 @@ struct fftset_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t cap_stream = nullptr;
-    hipStream_t side[2] = {nullptr, nullptr};
-    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace fsa {
@@ -252,27 +254,31 @@ static int commit(fftset_batch* b, hipStream_t st)
             group.insert(group.end(), g3[q].begin(), g3[q].end());
         }
     }
+    // fused tables: LDS tiles (all in phase 0) and four-step tiles, each as
+    // one range per workgroup size, keeping the largest-first order
     std::vector<RTile> fused;
-    for (int q = 0; q < 3; ++q) {
-        std::stable_sort(ph[q].begin(), ph[q].end(),
-                         [](const CostTile& x, const CostTile& y) { return x.cost > y.cost; });
-        // one range per workgroup size, each keeping the largest-first order
+    auto add_ranges = [&](const std::vector<CostTile>& src, bool want_lds, std::vector<TileRange>& dst) {
         for (int w = 256; w <= 1024; w *= 2) {
             TileRange r{(int)fused.size(), 0, w, 0};
-            for (auto& ct : ph[q]) {
+            for (auto& ct : src) {
                 const BatchClass& k = b->classes[ct.t.cls];
-                if (k.threads != w)
+                if (k.threads != w || (ct.t.kind == TK_LDS) != want_lds)
                     continue;
                 fused.push_back(ct.t);
                 r.count++;
                 r.lds_bytes = std::max(r.lds_bytes, class_lds_bytes(k));
             }
             if (r.count) {
-                b->phase[q].push_back(r);
+                dst.push_back(r);
                 b->lds_bytes = std::max(b->lds_bytes, r.lds_bytes);
             }
         }
-    }
+    };
+    for (int q = 0; q < 3; ++q)
+        std::stable_sort(ph[q].begin(), ph[q].end(),
+                         [](const CostTile& x, const CostTile& y) { return x.cost > y.cost; });
+    add_ranges(ph[0], true, b->lds_r);
+    for (int q = 0; q < 3; ++q) add_ranges(ph[q], false, b->fs_r[q]);
     for (size_t v = 0; v < job_tiles.size(); ++v) {
         const auto& tl = job_tiles[v];
         TileRange r{(int)per_job.size(), (int)tl.size(), 256, 0};
@@ -358,33 +364,70 @@ static int launch_range(const fftset_batch* b, const RTile* tiles, TileRange r, 
     return launch_ragged(a, r.threads, r.lds_bytes, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
 }
 
-// Phases run in order; within a phase the ranges of different workgroup
-// sizes are independent and, when side streams are given, run concurrently
-// (fork / join through events, which graph capture turns into branches).
+static int launch_list(const fftset_batch* b, const std::vector<TileRange>& rs, hipStream_t st)
+{
+    for (const TileRange& r : rs) {
+        int rc = launch_range(b, b->d_tiles, r, st);
+        if (rc != FFTSET_OK)
+            return rc;
+    }
+    return FFTSET_OK;
+}
+
+// Launch schedules (the graph records the same structure):
+//  BRANCHES (default): the four-step chain (columns -> rows -> closing
+//    columns) on the main stream, each workgroup size of LDS tiles on a side
+//    branch forked at the start and joined at the end;
+//  PHASED: the LDS groups and the four-step column step run concurrently,
+//    then the rows, then the closing columns;
+//  SERIAL: every group one after the other, largest workgroups first.
+// Without side streams (direct launches) every schedule runs serially.
 static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
 {
-    for (int q = 0; q < 3; ++q) {
-        const auto& rs = b->phase[q];
-        if (rs.empty())
-            continue;
-        const int extra = branches ? std::min<int>((int)rs.size() - 1, 2) : 0;
-        if (extra > 0 && hipEventRecord(b->fork, st) != hipSuccess)
-            return FFTSET_E_DEVICE;
-        for (int i = 0; i < (int)rs.size(); ++i) {
-            hipStream_t s = st;
-            if (i >= 1 && i <= extra) {
-                s = b->side[i - 1];
-                if (hipStreamWaitEvent(s, b->fork, 0) != hipSuccess)
-                    return FFTSET_E_DEVICE;
-            }
-            int rc = launch_range(b, b->d_tiles, rs[i], s);
-            if (rc != FFTSET_OK)
+    const int sched = branches ? b->sched : FFTSET_BATCH_SCHED_SERIAL;
+    auto fork_to = [&](int i) -> hipStream_t {
+        return hipStreamWaitEvent(b->side[i], b->fork, 0) == hipSuccess ? b->side[i] : nullptr;
+    };
+    auto join_from = [&](int i) {
+        return hipEventRecord(b->join[i], b->side[i]) == hipSuccess &&
+               hipStreamWaitEvent(st, b->join[i], 0) == hipSuccess;
+    };
+    if (sched == FFTSET_BATCH_SCHED_SERIAL) {
+        for (int i = (int)b->lds_r.size() - 1; i >= 0; --i)
+            if (int rc = launch_range(b, b->d_tiles, b->lds_r[i], st))
                 return rc;
-            if (i >= 1 && i <= extra &&
-                (hipEventRecord(b->join[i - 1], s) != hipSuccess || hipStreamWaitEvent(st, b->join[i - 1], 0) != hipSuccess))
-                return FFTSET_E_DEVICE;
-        }
+        for (int q = 0; q < 3; ++q)
+            if (int rc = launch_list(b, b->fs_r[q], st))
+                return rc;
+        return FFTSET_OK;
     }
+    const int nside = std::min<int>((int)b->lds_r.size(), 3);
+    if (hipEventRecord(b->fork, st) != hipSuccess)
+        return FFTSET_E_DEVICE;
+    for (int i = 0; i < (int)b->lds_r.size(); ++i) {
+        hipStream_t s = i < nside ? fork_to(i) : st;
+        if (!s)
+            return FFTSET_E_DEVICE;
+        if (int rc = launch_range(b, b->d_tiles, b->lds_r[i], s))
+            return rc;
+    }
+    if (sched == FFTSET_BATCH_SCHED_PHASED) {
+        if (int rc = launch_list(b, b->fs_r[0], st))
+            return rc;
+        for (int i = 0; i < nside; ++i)
+            if (!join_from(i))
+                return FFTSET_E_DEVICE;
+        for (int q = 1; q < 3; ++q)
+            if (int rc = launch_list(b, b->fs_r[q], st))
+                return rc;
+        return FFTSET_OK;
+    }
+    for (int q = 0; q < 3; ++q)
+        if (int rc = launch_list(b, b->fs_r[q], st))
+            return rc;
+    for (int i = 0; i < nside; ++i)
+        if (!join_from(i))
+            return FFTSET_E_DEVICE;
     return FFTSET_OK;
 }
 
@@ -441,6 +484,9 @@ int fftset_batch_commit(struct fftset_batch* b, unsigned flags)
         return rc;
     }
     b->threads = (flags & FFTSET_BATCH_THREADS_512) ? 512 : 256;
+    b->sched = (flags & FFTSET_BATCH_SCHED_SERIAL) ? FFTSET_BATCH_SCHED_SERIAL
+               : (flags & FFTSET_BATCH_SCHED_PHASED) ? FFTSET_BATCH_SCHED_PHASED
+                                                     : 0;
     b->use_graph = !(flags & FFTSET_BATCH_NO_GRAPH);
     hipStream_t st = thread_stream(b->ctx->device);
     rc = commit(b, st);
@@ -449,9 +495,11 @@ int fftset_batch_commit(struct fftset_batch* b, unsigned flags)
         bool ok = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) == hipSuccess &&
                   hipStreamCreateWithFlags(&b->side[0], hipStreamNonBlocking) == hipSuccess &&
                   hipStreamCreateWithFlags(&b->side[1], hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&b->side[2], hipStreamNonBlocking) == hipSuccess &&
                   hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
                   hipEventCreateWithFlags(&b->join[0], hipEventDisableTiming) == hipSuccess &&
-                  hipEventCreateWithFlags(&b->join[1], hipEventDisableTiming) == hipSuccess;
+                  hipEventCreateWithFlags(&b->join[1], hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&b->join[2], hipEventDisableTiming) == hipSuccess;
         if (!ok || hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             rc = FFTSET_E_DEVICE;
         } else {
@@ -520,8 +568,9 @@ int fftset_batch_info(const struct fftset_batch* b, int* ntiles, int* nclasses, 
         return FFTSET_E_INVALID;
     if (ntiles) {
         int n = 0;
+        for (auto& r : b->lds_r) n += r.count;
         for (int q = 0; q < 3; ++q)
-            for (auto& r : b->phase[q]) n += r.count;
+            for (auto& r : b->fs_r[q]) n += r.count;
         *ntiles = n;
     }
     if (nclasses)
@@ -547,7 +596,7 @@ void fftset_batch_destroy(struct fftset_batch* b)
         (void)hipGraphDestroy(b->graph);
     if (b->cap_stream)
         (void)hipStreamDestroy(b->cap_stream);
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 3; ++i) {
         if (b->side[i])
             (void)hipStreamDestroy(b->side[i]);
         if (b->join[i])

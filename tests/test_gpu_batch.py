@@ -83,7 +83,8 @@ SPEC = [
 ]
 
 
-@pytest.mark.parametrize("flags", [0, fftset_amd.BATCH_NO_GRAPH, fftset_amd.BATCH_THREADS_512])
+@pytest.mark.parametrize("flags", [0, fftset_amd.BATCH_NO_GRAPH, fftset_amd.BATCH_THREADS_512,
+                                   fftset_amd.BATCH_SCHED_SERIAL, fftset_amd.BATCH_SCHED_PHASED])
 def test_ragged_batch_matches_oracle(fs, oracle, flags):
     rng = np.random.default_rng(11 + flags)
     jobs = _make_jobs(rng, SPEC)
