@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
+    ap.add_argument("--tune-file", default=os.path.join(ROOT, "fftset_amd", "tune_mi355x.txt"),
+                    help="autotune table to plan with (default: the packaged MI355X table; '' for none)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device once (a known-size kernel for calibrating "
                          "rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -366,7 +368,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    fs = fftset_amd.FFTSet()
+    tune = args.tune_file if args.tune_file and os.path.exists(args.tune_file) else None
+    fs = fftset_amd.FFTSet(tune_file=tune)
     rng = np.random.default_rng(0)
     build = {"ragged": workload_ragged, "c2": workload_c2, "c3": workload_c3, "c5": workload_c5}[args.workload]
     built = build(torch, fs, rng, args)
@@ -410,6 +413,7 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "tune_file": os.path.relpath(tune, ROOT) if tune else None,
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "max_rel_rms_vs_oracle": err,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

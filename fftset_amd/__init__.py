@@ -98,6 +98,12 @@ def load_library():
     lib.fftset_batch_info.restype = ctypes.c_int
     lib.fftset_batch_destroy.argtypes = [vp]
     lib.fftset_batch_destroy.restype = None
+    lib.fftset_autotune.argtypes = [vp, vp, u, ctypes.POINTER(ctypes.c_float)]
+    lib.fftset_autotune.restype = ctypes.c_int
+    lib.fftset_tune_save.argtypes = [vp, ctypes.c_char_p]
+    lib.fftset_tune_save.restype = ctypes.c_int
+    lib.fftset_tune_load.argtypes = [vp, ctypes.c_char_p]
+    lib.fftset_tune_load.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -296,13 +302,15 @@ class FFTSet:
     FREQ_OFFSET_REAL = "FFTSET_MODULATION_FREQ_OFFSET_REAL"
     COMPLEX = "FFTSET_MODULATION_COMPLEX"
 
-    def __init__(self):
+    def __init__(self, tune_file=None):
         lib = load_library()
         self._mem = (ctypes.c_void_p * 8)()
         if lib.fftset_init(ctypes.addressof(self._mem)) != 0:
             raise FFTSetError(E_DEVICE, "fftset_init failed")
         self._alive = True
         self._plans = {}
+        if tune_file:
+            self.tune_load(tune_file)
 
     @property
     def address(self):
@@ -323,6 +331,26 @@ class FFTSet:
 
     def batch(self):
         return Batch(self)
+
+    def autotune(self, modulation, complex_bins):
+        """Times candidate chains / packings / splits for a size not yet
+        planned on this set; returns the winner's forward+conv ms per 64 MiB."""
+        ms = ctypes.c_float()
+        rc = load_library().fftset_autotune(self.address, _modulation(modulation), int(complex_bins),
+                                            ctypes.byref(ms))
+        if rc != OK:
+            raise FFTSetError(rc, f"autotune({modulation}, {complex_bins})")
+        return ms.value
+
+    def tune_save(self, path):
+        rc = load_library().fftset_tune_save(self.address, os.fsencode(path))
+        if rc != OK:
+            raise FFTSetError(rc, "tune_save")
+
+    def tune_load(self, path):
+        rc = load_library().fftset_tune_load(self.address, os.fsencode(path))
+        if rc != OK:
+            raise FFTSetError(rc, "tune_load")
 
     def destroy(self):
         if getattr(self, "_alive", False):

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -121,14 +122,23 @@ static int attach_modulation(fftset_fft* p, Ctx* ctx, hipStream_t st)
     return p->tmod ? FFTSET_OK : FFTSET_E_NOMEM;
 }
 
-static int plan_lds(fftset_fft* p, Ctx* ctx, hipStream_t st)
+int plan_lds_with(fftset_fft* p, Ctx* ctx, hipStream_t st, const std::vector<int>* chain, int G)
 {
     const unsigned L = p->lfft;
-    std::vector<int> radices = default_chain(L);
+    std::vector<int> radices = chain && !chain->empty() ? *chain : default_chain(L);
     if (!build_chain(radices, L, &p->chain))
         return FFTSET_E_UNSUPPORTED;
     if (!choose_shape(p->chain, 16, 0, &p->shape) && !choose_shape(p->chain, 32, 0, &p->shape))
         return FFTSET_E_UNSUPPORTED;
+    if (G > 0) {
+        // requested packing, kept within the lane and LDS limits
+        const int max_threads = p->shape.E <= 16 ? 1024 : 512;
+        const int lds_one = lds_len((int)L) * 8;
+        while (G > 1 && (G * p->shape.T > max_threads || G * lds_one > kMaxLdsBytes)) --G;
+        p->shape.G = G;
+        p->shape.threads = ((G * p->shape.T + 63) / 64) * 64;
+        p->shape.lds_bytes = G * lds_one;
+    }
     if (!attach_twiddles(ctx, &p->chain, st))
         return FFTSET_E_NOMEM;
     int rc = attach_modulation(p, ctx, st);
@@ -136,6 +146,12 @@ static int plan_lds(fftset_fft* p, Ctx* ctx, hipStream_t st)
         return rc;
     p->engine = ENG_LDS;
     return FFTSET_OK;
+}
+
+static int plan_lds(fftset_fft* p, Ctx* ctx, hipStream_t st)
+{
+    const TuneEntry* t = find_tune(ctx, p->mod, p->lfft);
+    return plan_lds_with(p, ctx, st, t ? &t->chain : nullptr, t ? t->G : 0);
 }
 
 static int run_lds(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
@@ -190,15 +206,17 @@ static bool row_shape(const ChainD& c, int rows_avail, int max_threads, ShapeD* 
     return out->G * out->T <= max_threads;
 }
 
-int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st)
+int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st, unsigned prefer_A)
 {
     // the most balanced split L = A * B with A <= B that both kernels accept
     FourStep best{};
     double best_score = 1e30;
-    for (unsigned A = 2; A * A <= L; ++A) {
+    for (unsigned A = 2; A < L; ++A) {
         if (L % A)
             continue;
         const unsigned B = L / A;
+        if (A > B && A != prefer_A)
+            continue;
         FourStep fs{};
         fs.A = A;
         fs.B = B;
@@ -208,7 +226,9 @@ int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStr
             continue;
         if (!col_shape(fs.colc, max_threads, &fs.cols) || !row_shape(fs.rowc, (int)A, max_threads, &fs.rows))
             continue;
-        const double score = std::abs(std::log2((double)B) - std::log2((double)A));
+        double score = std::abs(std::log2((double)B) - std::log2((double)A));
+        if (prefer_A && A == prefer_A)
+            score = -1.0;
         if (score < best_score) {
             best_score = score;
             best = fs;
@@ -225,9 +245,9 @@ int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStr
     return FFTSET_OK;
 }
 
-static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
+int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A)
 {
-    int rc = build_four_step(ctx, p->lfft, 1024, &p->fs, st);
+    int rc = build_four_step(ctx, p->lfft, 1024, &p->fs, st, A);
     if (rc != FFTSET_OK)
         return rc;
     rc = attach_modulation(p, ctx, st);
@@ -235,6 +255,12 @@ static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
         return rc;
     p->engine = ENG_FOUR_STEP;
     return FFTSET_OK;
+}
+
+static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
+{
+    const TuneEntry* t = find_tune(ctx, p->mod, p->lfft);
+    return plan_four_step_with(p, ctx, st, t ? t->A : 0);
 }
 
 static int run_four_step(const fftset_fft* p, int op, long long count, float* out, long long out_st,
@@ -422,6 +448,8 @@ int fftset_init(struct fftset* fc)
         delete ctx;
         return -1;
     }
+    if (const char* tf = getenv("FFTSET_AMD_TUNE_FILE"))
+        (void)load_tune_file(ctx, tf);
     fc->impl = ctx;
     return 0;
 }
@@ -481,7 +509,9 @@ const struct fftset_fft* fftset_create_fft(struct fftset* fc, const struct fftse
     p->modulator = modulation;
     p->ctx = ctx;
     p->mod = modulation->kind;
-    int rc = fits_lds(complex_bins) ? plan_lds(p, ctx, st) : FFTSET_E_UNSUPPORTED;
+    const TuneEntry* tuned = find_tune(ctx, p->mod, complex_bins);
+    const bool tuned_four_step = tuned && tuned->A && tuned->chain.empty();
+    int rc = fits_lds(complex_bins) && !tuned_four_step ? plan_lds(p, ctx, st) : FFTSET_E_UNSUPPORTED;
     if (rc == FFTSET_E_UNSUPPORTED)
         rc = plan_four_step(p, ctx, st);
     if (rc != FFTSET_OK) {

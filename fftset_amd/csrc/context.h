@@ -38,8 +38,21 @@ struct FourStep {
     const float2* tfs;    // tfs[k1*B + n2] = W_n^{n2 k1}
 };
 
+// Autotuned choices for one (modulation, size): radix chain and packing of
+// the LDS engine, or the four-step split. Empty / zero fields keep the
+// defaults.
+struct TuneEntry {
+    int mod;
+    unsigned L;
+    std::vector<int> chain;
+    int G;       // transforms per workgroup (LDS engine)
+    unsigned A;  // four-step column length
+    float ms;    // measured score (forward + convolution), for reports
+};
+
 struct Ctx {
     int device = 0;
+    std::vector<TuneEntry> tune;
     std::vector<Table> tables;
     size_t table_bytes = 0;
     fftset_fft* first = nullptr;  // plans sorted by descending size (src/fftset.c:97-129)
@@ -74,8 +87,15 @@ int use_device(const Ctx* ctx);
 void set_error(int code);
 const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st);
 bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st);
-// Four-step split of L whose column and row tiles fit max_threads lanes.
-int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st);
+// Four-step split of L whose column and row tiles fit max_threads lanes;
+// prefer_A > 0 forces that column length when it is valid.
+int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st, unsigned prefer_A = 0);
+// Plans p (mod, lfft, ctx set) with an optional radix chain / packing
+// (LDS engine) or split (four-step) instead of the defaults.
+int plan_lds_with(fftset_fft* p, Ctx* ctx, hipStream_t st, const std::vector<int>* chain, int G);
+int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A);
+const TuneEntry* find_tune(const Ctx* ctx, int mod, unsigned L);
+int load_tune_file(Ctx* ctx, const char* path);
 
 // Runs `op` over a uniform batch on `stream` (asynchronous). Returns an
 // FFTSET_* code.

@@ -60,6 +60,57 @@ std::vector<int> default_chain(unsigned n)
     return r;
 }
 
+static void chain_multisets(unsigned n, int start, int max_passes, std::vector<int>& cur,
+                            std::vector<std::vector<int>>& out)
+{
+    static const int kRadices[] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+    if (n == 1) {
+        if (!cur.empty())
+            out.push_back(cur);
+        return;
+    }
+    if ((int)cur.size() >= max_passes)
+        return;
+    for (int i = start; i < (int)(sizeof(kRadices) / sizeof(kRadices[0])); ++i) {
+        const int r = kRadices[i];
+        if (n % r)
+            continue;
+        // at most one radix-2 pass, one radix-5 family pass (5 | n once)
+        if (r == 2 && std::count(cur.begin(), cur.end(), 2))
+            continue;
+        cur.push_back(r);
+        chain_multisets(n / r, i, max_passes, cur, out);
+        cur.pop_back();
+    }
+}
+
+std::vector<std::vector<int>> candidate_chains(unsigned n, int max_passes, int limit)
+{
+    std::vector<std::vector<int>> sets, out;
+    std::vector<int> cur;
+    int a, b, c;
+    if (!factor235(n, &a, &b, &c))
+        return out;
+    out.push_back(default_chain(n));
+    chain_multisets(n, 0, max_passes, cur, sets);
+    // fewest passes first, then largest leading radix
+    std::stable_sort(sets.begin(), sets.end(), [](const std::vector<int>& x, const std::vector<int>& y) {
+        return x.size() < y.size();
+    });
+    for (auto& m : sets) {
+        std::vector<int> desc = m, asc = m;
+        std::sort(desc.begin(), desc.end(), [](int x, int y) { return x > y; });
+        std::sort(asc.begin(), asc.end());
+        for (auto* v : {&desc, &asc}) {
+            if ((int)out.size() >= limit)
+                return out;
+            if (std::find(out.begin(), out.end(), *v) == out.end())
+                out.push_back(*v);
+        }
+    }
+    return out;
+}
+
 bool build_chain(const std::vector<int>& radices, unsigned n, ChainD* out)
 {
     if (radices.empty() || (int)radices.size() > kMaxPasses)

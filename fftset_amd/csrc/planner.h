@@ -16,6 +16,12 @@ bool factor235(unsigned n, int* a, int* b, int* c);
 // from {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2}, largest first.
 std::vector<int> default_chain(unsigned n);
 
+// Candidate chains for the autotuner: every multiset of register radices
+// with product n and at most `max_passes` passes (at most one radix-2
+// pass), each in largest-first and smallest-first order; the default chain
+// comes first. At most `limit` chains.
+std::vector<std::vector<int>> candidate_chains(unsigned n, int max_passes, int limit);
+
 // Fills a ChainD for `radices` over length n (twiddle pointers left null).
 bool build_chain(const std::vector<int>& radices, unsigned n, ChainD* out);
 

@@ -195,6 +195,22 @@ int fftset_batch_info(const struct fftset_batch *b, int *ntiles, int *nclasses, 
 
 void fftset_batch_destroy(struct fftset_batch *b);
 
+/* --------------------------------------------------------------- tuning
+ * The radix chain, transforms per workgroup and (for sizes beyond LDS) the
+ * four-step split of a size are picked by timing candidates on the device.
+ * A result applies to plans created afterwards on the same set (a plan's
+ * kernel spectrum layout depends on its chain, so plans never change):
+ * call fftset_autotune before fftset_create_fft for that size. Tables
+ * persist as text; FFTSET_AMD_TUNE_FILE names one loaded by fftset_init. */
+
+/* Times the candidates for (modulation, complex_bins) and records the
+ * fastest; *best_ms (may be NULL) receives its forward+convolution time per
+ * 64 MiB of transforms. Fails with FFTSET_E_INVALID if the plan exists. */
+int fftset_autotune(struct fftset *fc, const struct fftset_modulation *modulation, unsigned complex_bins,
+                    float *best_ms);
+int fftset_tune_save(const struct fftset *fc, const char *path);
+int fftset_tune_load(struct fftset *fc, const char *path);
+
 /* Plan introspection (for tests, benchmarks and the autotuner). */
 unsigned fftset_fft_complex_bins(const struct fftset_fft *fft);
 /* Writes at most `max` radices of the plan's LDS chain; returns the count

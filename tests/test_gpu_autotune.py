@@ -1,0 +1,48 @@
+"""The autotuner: picks a measured chain / packing / split, later plans use
+it, results stay at parity, and tables round-trip through a file."""
+import numpy as np
+import pytest
+
+try:  # import at collection: the first import on a fresh box is slow
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+import fftset_amd
+from oracle import fp64
+from oracle.oracle import MOD_COMPLEX as OCX, MOD_FREQ_OFFSET_REAL as OFO
+from tests.gpu_util import gpu_conv, gpu_forward, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mod,L", [(fftset_amd.COMPLEX, 3840), (fftset_amd.FREQ_OFFSET_REAL, 4608),
+                                   (fftset_amd.FREQ_OFFSET_REAL, 61440)])
+def test_autotune_then_plan(oracle, tmp_path, mod, L):
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    fs = fftset_amd.FFTSet()
+    try:
+        ms = fs.autotune(mod, L)
+        assert ms > 0
+        f = fs.create_fft(mod, L)
+        radices = f.radices()
+        with pytest.raises(fftset_amd.FFTSetError):
+            fs.autotune(mod, L)  # plan exists: immutable
+        rng = np.random.default_rng(L)
+        x = rng.standard_normal(2 * L).astype(np.float32)
+        h = rng.standard_normal(2 * L).astype(np.float32)
+        o = oracle.plan(OFO if mod == fftset_amd.FREQ_OFFSET_REAL else OCX, L)
+        assert fp64.rel_rms(gpu_forward(f, x), o.forward(x)) <= 2e-6
+        assert fp64.rel_rms(gpu_conv(f, x, h), o.conv(x, o.get_kernel(h))) <= 2e-6
+        path = str(tmp_path / "tune.txt")
+        fs.tune_save(path)
+    finally:
+        fs.destroy()
+    # a fresh set that loads the table plans the same chain
+    fs2 = fftset_amd.FFTSet()
+    try:
+        fs2.tune_load(path)
+        assert fs2.create_fft(mod, L).radices() == radices
+    finally:
+        fs2.destroy()
