@@ -98,7 +98,7 @@ def load_library():
     lib.fftset_batch_info.restype = ctypes.c_int
     lib.fftset_batch_destroy.argtypes = [vp]
     lib.fftset_batch_destroy.restype = None
-    lib.fftset_autotune.argtypes = [vp, vp, u, ctypes.POINTER(ctypes.c_float)]
+    lib.fftset_autotune.argtypes = [vp, vp, u, u, ctypes.POINTER(ctypes.c_float)]
     lib.fftset_autotune.restype = ctypes.c_int
     lib.fftset_tune_save.argtypes = [vp, ctypes.c_char_p]
     lib.fftset_tune_save.restype = ctypes.c_int
@@ -332,11 +332,15 @@ class FFTSet:
     def batch(self):
         return Batch(self)
 
-    def autotune(self, modulation, complex_bins):
+    def autotune(self, modulation, complex_bins, ops=(OP_FORWARD, OP_CONV)):
         """Times candidate chains / packings / splits for a size not yet
-        planned on this set; returns the winner's forward+conv ms per 64 MiB."""
+        planned on this set on the given operations; returns the winner's
+        ms per 64 MiB."""
         ms = ctypes.c_float()
-        rc = load_library().fftset_autotune(self.address, _modulation(modulation), int(complex_bins),
+        mask = 0
+        for op in ops:
+            mask |= 1 << op
+        rc = load_library().fftset_autotune(self.address, _modulation(modulation), int(complex_bins), mask,
                                             ctypes.byref(ms))
         if rc != OK:
             raise FFTSetError(rc, f"autotune({modulation}, {complex_bins})")

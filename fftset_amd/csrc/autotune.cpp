@@ -4,9 +4,12 @@
 // (src/fftset_vec.c:1004-1062). Here the candidates are timed on the device:
 // for a size that fits LDS, register-radix chains (candidate_chains) crossed
 // with transforms-per-workgroup packings; for a larger size, the four-step
-// splits closest to balanced. The score is the time of one forward plus one
-// convolution over a batch of about 64 MiB. The winner is recorded per
-// (modulation, size) and applies to plans created afterwards: the kernel
+// splits closest to balanced. The score is the time of the requested
+// operations (forward and/or convolution, one launch each) over a batch of
+// about 512 MiB (large enough that launch overhead and clock ramp-up do not
+// decide the ranking), best of 3 trials.
+// The winner is recorded per (modulation, size) and applies to plans
+// created afterwards: the kernel
 // spectrum layout depends on the chain, so a plan never changes once made.
 // Tables persist as text, one entry per line:
 //   <mod> <L> <G> <A> <ms> <n> <r1> ... <rn>
@@ -98,9 +101,10 @@ struct Buffers {
     }
 };
 
-// Times forward + convolution of a temporary plan built for `c`.
-static int score(Ctx* ctx, int mod, unsigned L, const Cand& c, Buffers& b, long long count, hipStream_t st,
-                 float* ms)
+// Times the operations in `ops` (bit FFTSET_OP_*) of a temporary plan built
+// for `c`.
+static int score(Ctx* ctx, int mod, unsigned L, const Cand& c, Buffers& b, long long count, unsigned ops,
+                 hipStream_t st, float* ms)
 {
     fftset_fft p{};
     p.lfft = L;
@@ -113,9 +117,10 @@ static int score(Ctx* ctx, int mod, unsigned L, const Cand& c, Buffers& b, long 
         return FFTSET_E_UNSUPPORTED;
     const long long n2 = 2LL * L;
     auto run = [&]() {
-        int r = execute(&p, FFTSET_OP_FORWARD, count, b.out, n2, b.in, n2, nullptr, 0, b.work, n2, st);
-        if (r == FFTSET_OK)
-            r = execute(&p, FFTSET_OP_CONV, count, b.out, n2, b.in, n2, b.kern, n2, b.work, n2, st);
+        int r = FFTSET_OK;
+        for (int op = FFTSET_OP_FORWARD; op <= FFTSET_OP_CONV_GRAD_INPUT && r == FFTSET_OK; ++op)
+            if (ops & (1u << op))
+                r = execute(&p, op, count, b.out, n2, b.in, n2, b.kern, n2, b.work, n2, st);
         return r;
     };
     hipEvent_t e0, e1;
@@ -126,18 +131,24 @@ static int score(Ctx* ctx, int mod, unsigned L, const Cand& c, Buffers& b, long 
         return FFTSET_E_DEVICE;
     }
     rc = run();  // warm-up (first launch of each kernel)
+    if (rc == FFTSET_OK)
+        rc = run();
     const int reps = 3;
-    if (rc == FFTSET_OK && hipEventRecord(e0, st) != hipSuccess)
-        rc = FFTSET_E_DEVICE;
-    for (int i = 0; i < reps && rc == FFTSET_OK; ++i) rc = run();
-    if (rc == FFTSET_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
-        rc = FFTSET_E_DEVICE;
-    float t = 0.f;
-    if (rc == FFTSET_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess)
-        rc = FFTSET_E_DEVICE;
+    float best = 1e30f;
+    for (int trial = 0; trial < 3 && rc == FFTSET_OK; ++trial) {
+        if (hipEventRecord(e0, st) != hipSuccess)
+            rc = FFTSET_E_DEVICE;
+        for (int i = 0; i < reps && rc == FFTSET_OK; ++i) rc = run();
+        if (rc == FFTSET_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+            rc = FFTSET_E_DEVICE;
+        float t = 0.f;
+        if (rc == FFTSET_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+            rc = FFTSET_E_DEVICE;
+        best = std::min(best, t / reps);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    *ms = t / reps;
+    *ms = best;
     return rc;
 }
 
@@ -149,8 +160,15 @@ using namespace fsa;
 extern "C" {
 
 int fftset_autotune(struct fftset* fc, const struct fftset_modulation* modulation, unsigned complex_bins,
-                    float* best_ms)
+                    unsigned ops, float* best_ms)
 {
+    if (ops == 0)
+        ops = (1u << FFTSET_OP_FORWARD) | (1u << FFTSET_OP_CONV);
+    if (ops & ~((1u << FFTSET_OP_FORWARD) | (1u << FFTSET_OP_INVERSE) | (1u << FFTSET_OP_GET_KERNEL) |
+                (1u << FFTSET_OP_CONV) | (1u << FFTSET_OP_CONV_GRAD_INPUT))) {
+        set_error(FFTSET_E_INVALID);
+        return FFTSET_E_INVALID;
+    }
     if (!fc || !fc->impl || !modulation || (modulation->kind != MOD_FO && modulation->kind != MOD_CX)) {
         set_error(FFTSET_E_INVALID);
         return FFTSET_E_INVALID;
@@ -203,9 +221,9 @@ int fftset_autotune(struct fftset* fc, const struct fftset_modulation* modulatio
         for (size_t i = 0; i < splits.size() && i < 5; ++i) cands.push_back({{}, 0, splits[i].second});
     }
 
-    // buffers: about 64 MiB of transforms
+    // buffers: about 512 MiB of transforms
     const long long n2 = 2LL * complex_bins;
-    const long long count = std::max<long long>(64, (64LL << 20) / (n2 * 4));
+    const long long count = std::max<long long>(256, (512LL << 20) / (n2 * 4));
     Buffers b;
     const size_t bytes = (size_t)(count * n2) * sizeof(float);
     if (hipMalloc((void**)&b.in, bytes) != hipSuccess || hipMalloc((void**)&b.out, bytes) != hipSuccess ||
@@ -222,9 +240,9 @@ int fftset_autotune(struct fftset* fc, const struct fftset_modulation* modulatio
     best.L = complex_bins;
     for (const Cand& c : cands) {
         float ms = 0.f;
-        if (score(ctx, mod, complex_bins, c, b, count, st, &ms) != FFTSET_OK)
+        if (score(ctx, mod, complex_bins, c, b, count, ops, st, &ms) != FFTSET_OK)
             continue;
-        const float norm = ms * (64.f * (1 << 20)) / (float)(count * n2 * 4);  // per 64 MiB
+        const float norm = ms * (64.f * (1 << 20)) / (float)(count * n2 * 4);  // reported per 64 MiB
         if (norm < best.ms) {
             best.ms = norm;
             best.chain = c.chain;

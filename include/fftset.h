@@ -203,11 +203,12 @@ void fftset_batch_destroy(struct fftset_batch *b);
  * call fftset_autotune before fftset_create_fft for that size. Tables
  * persist as text; FFTSET_AMD_TUNE_FILE names one loaded by fftset_init. */
 
-/* Times the candidates for (modulation, complex_bins) and records the
- * fastest; *best_ms (may be NULL) receives its forward+convolution time per
- * 64 MiB of transforms. Fails with FFTSET_E_INVALID if the plan exists. */
+/* Times the candidates for (modulation, complex_bins) on the operations in
+ * `ops` (a mask of 1 << FFTSET_OP_*; 0 = forward + convolution) and records
+ * the fastest; *best_ms (may be NULL) receives its time per 64 MiB of
+ * transforms. Fails with FFTSET_E_INVALID if the plan exists. */
 int fftset_autotune(struct fftset *fc, const struct fftset_modulation *modulation, unsigned complex_bins,
-                    float *best_ms);
+                    unsigned ops, float *best_ms);
 int fftset_tune_save(const struct fftset *fc, const char *path);
 int fftset_tune_load(struct fftset *fc, const char *path);
 

@@ -46,3 +46,21 @@ def test_autotune_then_plan(oracle, tmp_path, mod, L):
         assert fs2.create_fft(mod, L).radices() == radices
     finally:
         fs2.destroy()
+
+
+def test_autotune_ops_mask(oracle):
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    fs = fftset_amd.FFTSet()
+    try:
+        with pytest.raises(fftset_amd.FFTSetError):
+            fs.autotune(fftset_amd.FREQ_OFFSET_REAL, 768, ops=(fftset_amd.OP_CONV_GRAD_KERNEL,))
+        assert fs.autotune(fftset_amd.FREQ_OFFSET_REAL, 768, ops=(fftset_amd.OP_CONV,)) > 0
+        f = fs.create_fft(fftset_amd.FREQ_OFFSET_REAL, 768)
+        rng = np.random.default_rng(768)
+        x = rng.standard_normal(2 * 768).astype(np.float32)
+        h = rng.standard_normal(2 * 768).astype(np.float32)
+        o = oracle.plan(OFO, 768)
+        assert fp64.rel_rms(gpu_conv(f, x, h), o.conv(x, o.get_kernel(h))) <= 2e-6
+    finally:
+        fs.destroy()
