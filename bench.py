@@ -349,6 +349,21 @@ def cpu_baseline_uniform(x_dev, L, mod, ops, seconds, kern_dev=None, h_dev=None)
             "sample": f"{done} transforms x {'+'.join(ops)} at L={L}, {dt:.1f} s"}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprof PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration),
+    or None when no summary covers this workload and kernel."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return {"bytes_per_launch": int(k["hbm_bytes_per_launch"]), "source": os.path.relpath(path, ROOT)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -396,6 +411,7 @@ def main():
     value = total_bytes / (step_ms * 1e-3) / 1e9
     dom = max(ms, key=ms.get)
     achieved = bytes_per[dom] / (ms[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, dom)
     err = None if args.no_check else check()
     out = {
         "metric": METRIC,
@@ -417,7 +433,10 @@ def main():
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "max_rel_rms_vs_oracle": err,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "algorithmic_bytes": bytes_per[dom]},
         "cpu_baseline": None,
     }
     if baselines is not None:
