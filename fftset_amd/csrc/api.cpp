@@ -177,12 +177,15 @@ static int run_lds(const fftset_fft* p, int op, long long count, float* out, lon
 }
 
 // ------------------------------------------------------------- four-step
-static bool col_shape(const ChainD& c, int max_threads, ShapeD* out)
+// Column tiles: C adjacent columns per workgroup, so every global access is
+// a run of C values. 32 measured best on MI355X at 61440 = 240 x 256
+// (wider runs, still two or more workgroups per CU).
+static bool col_shape(const ChainD& c, int max_threads, int C0, ShapeD* out)
 {
     ShapeD one;
     if (!choose_shape(c, 16, 1, &one))
         return false;
-    int C = 16;
+    int C = C0 > 0 ? C0 : 32;
     while (C > 1 && (C * one.T > max_threads || (long long)C * c.n * 8 > kMaxLdsBytes)) C /= 2;
     if (C * one.T > max_threads || (long long)C * c.n * 8 > kMaxLdsBytes)
         return false;
@@ -206,7 +209,8 @@ static bool row_shape(const ChainD& c, int rows_avail, int max_threads, ShapeD* 
     return out->G * out->T <= max_threads;
 }
 
-int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st, unsigned prefer_A)
+int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st, unsigned prefer_A,
+                    int cols_C)
 {
     // the most balanced split L = A * B with A <= B that both kernels accept
     FourStep best{};
@@ -224,7 +228,8 @@ int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStr
             continue;
         if (!build_chain(default_chain(A), A, &fs.colc) || !build_chain(default_chain(B), B, &fs.rowc))
             continue;
-        if (!col_shape(fs.colc, max_threads, &fs.cols) || !row_shape(fs.rowc, (int)A, max_threads, &fs.rows))
+        if (!col_shape(fs.colc, max_threads, cols_C, &fs.cols) ||
+            !row_shape(fs.rowc, (int)A, max_threads, &fs.rows))
             continue;
         double score = std::abs(std::log2((double)B) - std::log2((double)A));
         if (prefer_A && A == prefer_A)
@@ -245,9 +250,9 @@ int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStr
     return FFTSET_OK;
 }
 
-int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A)
+int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A, int cols_C)
 {
-    int rc = build_four_step(ctx, p->lfft, 1024, &p->fs, st, A);
+    int rc = build_four_step(ctx, p->lfft, 1024, &p->fs, st, A, cols_C);
     if (rc != FFTSET_OK)
         return rc;
     rc = attach_modulation(p, ctx, st);
@@ -260,7 +265,7 @@ int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A)
 static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
 {
     const TuneEntry* t = find_tune(ctx, p->mod, p->lfft);
-    return plan_four_step_with(p, ctx, st, t ? t->A : 0);
+    return plan_four_step_with(p, ctx, st, t ? t->A : 0, t && t->chain.empty() ? t->G : 0);
 }
 
 static int run_four_step(const fftset_fft* p, int op, long long count, float* out, long long out_st,

@@ -4,10 +4,11 @@
 // (src/fftset_vec.c:1004-1062). Here the candidates are timed on the device:
 // for a size that fits LDS, register-radix chains (candidate_chains) crossed
 // with transforms-per-workgroup packings; for a larger size, the four-step
-// splits closest to balanced. The score is the time of the requested
-// operations (forward and/or convolution, one launch each) over a batch of
-// about 512 MiB (large enough that launch overhead and clock ramp-up do not
-// decide the ranking), best of 3 trials.
+// splits closest to balanced, each with 16, 32 or 64 columns per column tile
+// (stored in the G field of the table). The score is the time of the
+// requested operations (forward and/or convolution, one launch each) over a
+// batch of about 512 MiB (large enough that launch overhead and clock
+// ramp-up do not decide the ranking), best of 3 trials.
 // The winner is recorded per (modulation, size) and applies to plans
 // created afterwards: the kernel
 // spectrum layout depends on the chain, so a plan never changes once made.
@@ -110,10 +111,10 @@ static int score(Ctx* ctx, int mod, unsigned L, const Cand& c, Buffers& b, long 
     p.lfft = L;
     p.ctx = ctx;
     p.mod = mod;
-    int rc = c.A ? plan_four_step_with(&p, ctx, st, c.A) : plan_lds_with(&p, ctx, st, &c.chain, c.G);
+    int rc = c.A ? plan_four_step_with(&p, ctx, st, c.A, c.G) : plan_lds_with(&p, ctx, st, &c.chain, c.G);
     if (rc != FFTSET_OK)
         return rc;
-    if (c.A && p.fs.A != c.A)
+    if (c.A && (p.fs.A != c.A || p.fs.cols.G != c.G))
         return FFTSET_E_UNSUPPORTED;
     const long long n2 = 2LL * L;
     auto run = [&]() {
@@ -218,7 +219,8 @@ int fftset_autotune(struct fftset* fc, const struct fftset_modulation* modulatio
             if (complex_bins % A == 0 && fits_lds(A) && fits_lds(complex_bins / A))
                 splits.push_back({std::abs(std::log2((double)(complex_bins / A)) - std::log2((double)A)), A});
         std::sort(splits.begin(), splits.end());
-        for (size_t i = 0; i < splits.size() && i < 5; ++i) cands.push_back({{}, 0, splits[i].second});
+        for (size_t i = 0; i < splits.size() && i < 4; ++i)
+            for (int C : {16, 32, 64}) cands.push_back({{}, C, splits[i].second});
     }
 
     // buffers: about 512 MiB of transforms

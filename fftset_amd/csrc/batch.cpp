@@ -38,7 +38,8 @@ struct BatchClass {
     RClass rc;              // device descriptor (pointers patched at upload)
     bool four_step;
     FourStep fs;
-    int threads;            // workgroup size of this class's tiles
+    int threads;            // workgroup size of this class's LDS tiles
+    int col_threads, row_threads;  // four-step: column and row tiles
 };
 
 // A contiguous run of tiles launched together with one workgroup size.
@@ -146,18 +147,34 @@ static int plan_class(fftset_batch* b, BatchClass& k, hipStream_t)
     r.rowT = k.fs.rows.T;
     r.rowR = k.fs.rows.G;
     r.rowGroups = (r.A + r.rowR - 1) / r.rowR;
-    k.threads = round_threads(std::max(r.colC * r.colT, r.rowR * r.rowT), b->threads);
-    return k.threads < 0 ? FFTSET_E_UNSUPPORTED : FFTSET_OK;
+    // column and row tiles run in workgroups of their own size
+    k.col_threads = round_threads(r.colC * r.colT, b->threads);
+    k.row_threads = round_threads(r.rowR * r.rowT, b->threads);
+    k.threads = std::max(k.col_threads, k.row_threads);
+    return k.col_threads < 0 || k.row_threads < 0 ? FFTSET_E_UNSUPPORTED : FFTSET_OK;
+}
+
+// Workgroup size and LDS bytes of one tile kind of a class.
+static int tile_threads(const BatchClass& k, int kind)
+{
+    if (kind == TK_LDS)
+        return k.threads;
+    return kind == TK_ROW ? k.row_threads : k.col_threads;
+}
+
+static int tile_lds_bytes(const BatchClass& k, int kind)
+{
+    const RClass& r = k.rc;
+    if (kind == TK_LDS)
+        return r.G * lds_len(r.L) * 8;
+    return kind == TK_ROW ? r.rowR * lds_len(r.B) * 8 : r.colC * r.A * 8;
 }
 
 static int class_lds_bytes(const BatchClass& k)
 {
-    const RClass& r = k.rc;
     if (!k.four_step)
-        return r.G * lds_len(r.L) * 8;
-    const int col = r.colC * r.A * 8;
-    const int row = r.rowR * lds_len(r.B) * 8;
-    return std::max(col, row);
+        return tile_lds_bytes(k, TK_LDS);
+    return std::max(tile_lds_bytes(k, TK_COL), tile_lds_bytes(k, TK_ROW));
 }
 
 struct CostTile {
@@ -250,7 +267,9 @@ static int commit(fftset_batch* b, hipStream_t st)
         for (int q = 0; q < 3; ++q) {
             if (g3[q].empty())
                 continue;
-            b->group_ranges.push_back({(int)group.size(), (int)g3[q].size(), k.threads, class_lds_bytes(k)});
+            const int kind = g3[q][0].kind;
+            b->group_ranges.push_back(
+                {(int)group.size(), (int)g3[q].size(), tile_threads(k, kind), tile_lds_bytes(k, kind)});
             group.insert(group.end(), g3[q].begin(), g3[q].end());
         }
     }
@@ -262,11 +281,11 @@ static int commit(fftset_batch* b, hipStream_t st)
             TileRange r{(int)fused.size(), 0, w, 0};
             for (auto& ct : src) {
                 const BatchClass& k = b->classes[ct.t.cls];
-                if (k.threads != w || (ct.t.kind == TK_LDS) != want_lds)
+                if (tile_threads(k, ct.t.kind) != w || (ct.t.kind == TK_LDS) != want_lds)
                     continue;
                 fused.push_back(ct.t);
                 r.count++;
-                r.lds_bytes = std::max(r.lds_bytes, class_lds_bytes(k));
+                r.lds_bytes = std::max(r.lds_bytes, tile_lds_bytes(k, ct.t.kind));
             }
             if (r.count) {
                 dst.push_back(r);
@@ -284,8 +303,8 @@ static int commit(fftset_batch* b, hipStream_t st)
         TileRange r{(int)per_job.size(), (int)tl.size(), 256, 0};
         if (!tl.empty()) {
             const BatchClass& k = b->classes[tl[0].cls];
-            r.threads = k.threads;
-            r.lds_bytes = class_lds_bytes(k);
+            r.threads = tile_threads(k, tl[0].kind);
+            r.lds_bytes = tile_lds_bytes(k, tl[0].kind);
         }
         b->job_ranges.push_back(r);
         per_job.insert(per_job.end(), tl.begin(), tl.end());

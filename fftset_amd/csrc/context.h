@@ -45,9 +45,9 @@ struct TuneEntry {
     int mod;
     unsigned L;
     std::vector<int> chain;
-    int G;       // transforms per workgroup (LDS engine)
+    int G;       // transforms per workgroup (LDS engine) or columns per column tile (four-step)
     unsigned A;  // four-step column length
-    float ms;    // measured score (forward + convolution), for reports
+    float ms;    // measured score of the tuned operations, for reports
 };
 
 struct Ctx {
@@ -89,11 +89,13 @@ const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipSt
 bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st);
 // Four-step split of L whose column and row tiles fit max_threads lanes;
 // prefer_A > 0 forces that column length when it is valid.
-int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st, unsigned prefer_A = 0);
+// cols_C: columns per column tile (0 = default).
+int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStream_t st, unsigned prefer_A = 0,
+                    int cols_C = 0);
 // Plans p (mod, lfft, ctx set) with an optional radix chain / packing
 // (LDS engine) or split (four-step) instead of the defaults.
 int plan_lds_with(fftset_fft* p, Ctx* ctx, hipStream_t st, const std::vector<int>* chain, int G);
-int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A);
+int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A, int cols_C = 0);
 const TuneEntry* find_tune(const Ctx* ctx, int mod, unsigned L);
 int load_tune_file(Ctx* ctx, const char* path);
 
