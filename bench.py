@@ -17,6 +17,9 @@ Workloads (complex_bins = L):
   c2       COMPLEX L=3840 x 65536, forward + inverse
   c3       FREQOFFSETREAL conv L=4096 x 16384 channels
   c5       FREQOFFSETREAL conv L=2048 x 4096 + both adjoints
+  block    overlap-save FIR filter, 1024 taps, 64 channels x 4 Mi samples
+           (fftset_recommend_conv_length(1024, 0) = 8192 bins; algorithmic
+           bytes: 8 per sample, signal in and output out)
 """
 import argparse
 import json
@@ -37,7 +40,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="ragged", choices=["ragged", "c2", "c3", "c5"])
+    ap.add_argument("--workload", default="ragged", choices=["ragged", "c2", "c3", "c5", "block"])
     ap.add_argument("--transforms", type=int, default=20000, help="ragged: batch size")
     ap.add_argument("--threads", type=int, default=256, choices=[256, 512], help="ragged: lanes per workgroup")
     ap.add_argument("--no-graph", action="store_true", help="ragged: launch phases directly")
@@ -174,6 +177,71 @@ def workload_c5(torch, fs, rng_np, args):
         return float(e)
 
     return kernels, bytes_per, cfg, check, None
+
+
+def workload_block(torch, fs, rng_np, args):
+    import numpy as np
+    import fftset_amd
+    k, C, n = 1024, 64, 1 << 22
+    g = torch.Generator(device="cuda").manual_seed(6)
+    t = torch.arange(k, device="cuda", dtype=torch.float32)
+    h = torch.randn(k, device="cuda", generator=g) * torch.exp(-t / 256.0)
+    x = torch.rand(C, n, device="cuda", generator=g) * 2 - 1
+    y = torch.empty_like(x)
+    bc = fs.blockconv(h, 0)
+    kernels = {"block_conv": lambda: bc.run(y, x)}
+    bytes_per = {"block_conv": 8 * C * n}
+    cfg = {"workload": "overlap-save FIR filter of long real signals", "taps": k, "channels": C,
+           "samples_per_channel": n, "complex_bins": bc.complex_bins, "block_size": bc.block_size}
+
+    def check():
+        from oracle import fp64
+        m = 100000
+        hv = h.cpu().numpy().astype(np.float64)
+        worst = 0.0
+        for c in (0, C - 1):
+            xs = x[c, :m].cpu().numpy().astype(np.float64)
+            ref = np.convolve(xs, hv)[:m]
+            worst = max(worst, float(fp64.rel_rms(y[c, :m].cpu().numpy(), ref)))
+        return worst
+
+    def cpu_sample(seconds):
+        return cpu_baseline_block(x, h, bc.complex_bins, bc.block_size, seconds)
+
+    return kernels, bytes_per, cfg, check, cpu_sample
+
+
+def cpu_baseline_block(x_dev, h_dev, L, b, seconds):
+    """Oracle FREQOFFSETREAL convolution of overlap-save frames on the host
+    cores, one frame per thread, over the first blocks of channel 0 (frames
+    gathered outside the timed calls)."""
+    import numpy as np
+    from oracle import Oracle
+    from oracle.oracle import MOD_FREQ_OFFSET_REAL, OP_CONV
+    threads = min(16, os.cpu_count() or 1)
+    o = Oracle().plan(MOD_FREQ_OFFSET_REAL, L)
+    N = 2 * L
+    h = h_dev.cpu().numpy()
+    k = len(h)
+    hk = np.zeros(N, np.float32)
+    hk[:k] = h / np.float32(L)
+    K = o.get_kernel(hk)
+    nb = 4 * threads
+    x = x_dev[0, : nb * b].cpu().numpy()
+    frames = np.zeros((nb, N), np.float32)
+    for i in range(nb):
+        s0 = i * b - (k - 1)
+        lo, hi = max(0, s0), min(len(x), s0 + N)
+        frames[i, lo - s0: hi - s0] = x[lo:hi]
+    Ks = np.broadcast_to(K, frames.shape).copy()
+    busy, done = 0.0, 0
+    while busy < seconds:
+        t1 = time.perf_counter()
+        o.run_batch(OP_CONV, frames, kern=Ks, nthreads=threads)
+        busy += time.perf_counter() - t1
+        done += nb
+    return {"value": round(8 * b * done / busy / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{done} overlap-save blocks of {b} samples (channel 0), {busy:.1f} s of compute"}
 
 
 RAGGED_SIZES = [48, 120, 360, 768, 1440, 4608, 15360, 61440]
@@ -386,7 +454,8 @@ def main():
     tune = args.tune_file if args.tune_file and os.path.exists(args.tune_file) else None
     fs = fftset_amd.FFTSet(tune_file=tune)
     rng = np.random.default_rng(0)
-    build = {"ragged": workload_ragged, "c2": workload_c2, "c3": workload_c3, "c5": workload_c5}[args.workload]
+    build = {"ragged": workload_ragged, "c2": workload_c2, "c3": workload_c3, "c5": workload_c5,
+             "block": workload_block}[args.workload]
     built = build(torch, fs, rng, args)
     kernels, bytes_per, cfg, check, cpu_sample = built[:5]
     baselines = built[5] if len(built) > 5 else None

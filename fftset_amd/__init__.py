@@ -104,6 +104,16 @@ def load_library():
     lib.fftset_tune_save.restype = ctypes.c_int
     lib.fftset_tune_load.argtypes = [vp, ctypes.c_char_p]
     lib.fftset_tune_load.restype = ctypes.c_int
+    lib.fftset_blockconv_create.argtypes = [vp, fp, u, u]
+    lib.fftset_blockconv_create.restype = vp
+    lib.fftset_blockconv_run.argtypes = [vp, fp, sz, fp, sz, sz, sz, u, vp]
+    lib.fftset_blockconv_run.restype = ctypes.c_int
+    lib.fftset_blockconv_block_size.argtypes = [vp]
+    lib.fftset_blockconv_block_size.restype = u
+    lib.fftset_blockconv_complex_bins.argtypes = [vp]
+    lib.fftset_blockconv_complex_bins.restype = u
+    lib.fftset_blockconv_destroy.argtypes = [vp]
+    lib.fftset_blockconv_destroy.restype = None
     _lib = lib
     return lib
 
@@ -226,6 +236,7 @@ BATCH_SCHED_SERIAL = 4
 BATCH_SCHED_PHASED = 8
 BASELINE_PER_GROUP = 1
 BASELINE_PER_TRANSFORM = 2
+BLOCKCONV_HISTORY = 1
 
 
 def _stream_handle(stream):
@@ -296,6 +307,53 @@ class Batch:
             pass
 
 
+class BlockConv:
+    """``struct fftset_blockconv``: causal FIR filtering of long real signals
+    by overlap-save over FREQOFFSETREAL convolutions (one launch per run)."""
+
+    def __init__(self, fs, taps, max_block_size=0):
+        self._fs = fs
+        self.taps = int(taps.numel()) if hasattr(taps, "numel") else None
+        self.handle = load_library().fftset_blockconv_create(fs.address, _ptr(taps), self.taps, max_block_size)
+        if not self.handle:
+            raise FFTSetError(load_library().fftset_last_error(), "blockconv_create")
+
+    @property
+    def block_size(self):
+        return int(load_library().fftset_blockconv_block_size(self.handle))
+
+    @property
+    def complex_bins(self):
+        return int(load_library().fftset_blockconv_complex_bins(self.handle))
+
+    def run(self, out, inp, nsamples=None, channels=None, in_stride=None, out_stride=None, flags=0, stream=None):
+        """Filters every row of ``inp`` (float32 CUDA tensor, rows = channels,
+        or a device address with explicit sizes) into ``out``; with
+        BLOCKCONV_HISTORY the k-1 samples in front of each row (``in_stride``
+        apart) are the signal's past."""
+        if nsamples is None:
+            nsamples = out.shape[-1]
+        if channels is None:
+            channels = out.numel() // out.shape[-1] if out.dim() > 1 else 1
+        in_stride = nsamples if in_stride is None else in_stride
+        out_stride = nsamples if out_stride is None else out_stride
+        rc = load_library().fftset_blockconv_run(self.handle, _ptr(out), out_stride, _ptr(inp), in_stride, nsamples,
+                                                 channels, flags, _stream_handle(stream))
+        if rc != OK:
+            raise FFTSetError(rc, "blockconv_run")
+
+    def destroy(self):
+        if getattr(self, "handle", None):
+            load_library().fftset_blockconv_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class FFTSet:
     """``struct fftset``: owns plans and the HBM twiddle tables."""
 
@@ -331,6 +389,10 @@ class FFTSet:
 
     def batch(self):
         return Batch(self)
+
+    def blockconv(self, taps, max_block_size=0):
+        """Overlap-save FIR filter with the given taps (float32 CUDA tensor)."""
+        return BlockConv(self, taps, max_block_size)
 
     def autotune(self, modulation, complex_bins, ops=(OP_FORWARD, OP_CONV)):
         """Times candidate chains / packings / splits for a size not yet

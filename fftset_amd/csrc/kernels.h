@@ -83,6 +83,28 @@ struct RaggedArgs {
 // Launches tiles [0, ntiles) of `a`; every class must fit `threads` lanes.
 hipError_t launch_ragged(const RaggedArgs& a, int threads, int lds_bytes, hipStream_t st);
 
+// ------------------------------------------------------- block convolution
+// Overlap-save FIR filtering of long real signals (see kernels_block.hip).
+struct BlockArgs {
+    ChainD c;
+    const float2* tmod;
+    int L;               // complex bins; frames are N = 2L samples
+    long long ntrans;    // channels * nblocks frames
+    int G, T;
+    const float* in;     // channel c starts at in + c * in_st
+    float* out;          // channel c starts at out + c * out_st
+    const float* kern;   // shared kernel spectrum of the taps (scaled 1/L)
+    long long in_st, out_st;
+    long long n;         // samples per channel
+    long long nblocks;   // blocks per channel
+    long long b;         // new samples per block (N - k + 1)
+    int skip;            // k - 1
+    long long lo;        // first readable sample: 0, or -(k-1) with history
+};
+
+hipError_t launch_block(const BlockArgs& a, const ShapeD& sh, hipStream_t st);
+hipError_t launch_taps(float* dst, const float* taps, int k, int n, float scale, hipStream_t st);
+
 // Grid for a persistent kernel: workgroups resident at once, capped by the
 // number of tiles.
 long long persistent_grid(const void* fn, int threads, int lds_bytes, long long tiles);

@@ -1,0 +1,110 @@
+// kernels_block.hip -- block (overlap-save) FIR convolution of long real
+// signals through the FREQOFFSETREAL convolution engine.
+//
+// With N = 2L and k taps, a block carries b = N - k + 1 new samples. Frame
+// (channel c, block i) is the N input samples starting at
+// s0 = i*b - (k-1); its negacyclic convolution with h (k taps, the 1/L
+// normalisation folded into the kernel spectrum) has no wrap-around at
+// frame positions p >= k - 1, which are the outputs y[i*b + p - (k-1)].
+// The frame gather (with zero fill outside the signal) is fused into the
+// first pass and the valid-output scatter into the last pass, so the
+// signal and the output cross HBM once; only the k - 1 overlap samples per
+// block are read twice.
+#include <algorithm>
+
+#include "drivers.h"
+#include "kernels.h"
+
+namespace fsa {
+
+// First pass: frame sample s reads x[s] for lo <= s < n, else 0; then the
+// FREQOFFSETREAL modulation of LdFoMod.
+struct LdFrame {
+    const float* __restrict__ x;  // channel base (sample 0)
+    long long s0, lo, n;
+    int L;
+    const float2* __restrict__ tmod;
+    __device__ __forceinline__ float sample(long long s) const { return (s >= lo && s < n) ? x[s] : 0.f; }
+    __device__ __forceinline__ float2 operator()(int e) const
+    {
+        const float re = sample(s0 + e), im = sample(s0 + L + e);
+        const float2 w = tmod[e];
+        return make_float2(re * w.x + im * w.y, re * w.y - im * w.x);
+    }
+};
+
+// Last pass: the demodulation of StFoDemod, keeping frame positions
+// p >= k - 1 that land inside the signal.
+struct StFrame {
+    float* __restrict__ y;  // channel base (sample 0)
+    long long o0, n;        // output sample of frame position 0; signal length
+    int L, skip;            // skip = k - 1
+    const float2* __restrict__ tmod;
+    __device__ __forceinline__ void put(int p, float v) const
+    {
+        const long long o = o0 + p;
+        if (p >= skip && o < n)
+            y[o] = v;
+    }
+    __device__ __forceinline__ void operator()(int f, float2 v) const
+    {
+        const float2 w = tmod[f];
+        put(f, v.x * w.x - v.y * w.y);
+        put(L + f, v.x * w.y + v.y * w.x);
+    }
+};
+
+template <int E>
+__global__ void __launch_bounds__(1024) k_block(const BlockArgs a)
+{
+    extern __shared__ float2 smem[];
+    const int T = a.T, G = a.G;
+    const int g = threadIdx.x / T;
+    const int lt0 = threadIdx.x - g * T;
+    const bool in_tile = g < G;
+    const TileContig tl{smem + (in_tile ? g : 0) * lds_len(a.L)};
+    constexpr int kIdle = 1 << 28;
+
+    const long long tr = (long long)blockIdx.x * G + g;
+    const bool ok = in_tile && tr < a.ntrans;
+    const int lt = ok ? lt0 : kIdle;
+    const long long t0 = ok ? tr : 0;
+    const long long c = t0 / a.nblocks;
+    const long long i = t0 - c * a.nblocks;
+    const long long s0 = i * a.b - a.skip;
+    const LdFrame ld{a.in + c * a.in_st, s0, a.lo, a.n, a.L, a.tmod};
+    const StFrame st{a.out + c * a.out_st, s0, a.n, a.L, a.skip, a.tmod};
+    const BotConv bot{(const float2*)a.kern, false};
+    run_conv<E>(a.c, tl, lt, T, ld, st, bot);
+}
+
+hipError_t launch_block(const BlockArgs& a, const ShapeD& sh, hipStream_t st)
+{
+    if (sh.E != 16)
+        return hipErrorInvalidValue;
+    const long long tiles = (a.ntrans + sh.G - 1) / sh.G;
+    if (tiles <= 0)
+        return hipSuccess;
+    if (tiles > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    if (sh.lds_bytes > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_block<16>, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
+    hipLaunchKernelGGL(k_block<16>, dim3((unsigned)tiles), dim3(sh.threads), sh.lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+// taps (k floats) scaled by `scale` into the first k of n zeroed floats.
+__global__ void k_taps(float* __restrict__ dst, const float* __restrict__ taps, int k, int n, float scale)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = i < k ? taps[i] * scale : 0.f;
+}
+
+hipError_t launch_taps(float* dst, const float* taps, int k, int n, float scale, hipStream_t st)
+{
+    const int grid = std::min(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(k_taps, dim3(grid), dim3(256), 0, st, dst, taps, k, n, scale);
+    return hipGetLastError();
+}
+
+}  // namespace fsa

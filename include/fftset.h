@@ -212,6 +212,38 @@ int fftset_autotune(struct fftset *fc, const struct fftset_modulation *modulatio
 int fftset_tune_save(const struct fftset *fc, const char *path);
 int fftset_tune_load(struct fftset *fc, const char *path);
 
+/* ----------------------------------------------------- block convolution
+ * No counterpart in the reference: the overlap-save driver that the
+ * reference's fftset_recommend_conv_length (fftset.h:144-151) sizes
+ * transforms for. Causal FIR filtering of long real signals,
+ *   y[n] = sum_{j<k} h[j] x[n-j],  0 <= n < nsamples,
+ * with x[n] = 0 for n < 0 (or, with FFTSET_BLOCKCONV_HISTORY, read from the
+ * k-1 samples the caller keeps in front of every channel). The transform
+ * has L = fftset_recommend_conv_length(k, max_block_size) complex bins;
+ * every block of 2L - k + 1 samples of every channel is one FREQOFFSETREAL
+ * convolution, all of them in one launch. Sizes with L > 16384 are not
+ * supported (FFTSET_E_UNSUPPORTED). */
+struct fftset_blockconv;
+
+#define FFTSET_BLOCKCONV_HISTORY 1u  /* in[c][-(k-1) .. -1] hold the samples before the block */
+
+/* taps: k floats in device memory (read once). The set must outlive the
+ * object. NULL on failure (see fftset_last_error). */
+struct fftset_blockconv *fftset_blockconv_create(struct fftset *fc, const float *taps, unsigned kernel_length,
+                                                 unsigned max_block_size);
+
+/* Filters `channels` signals of `nsamples` samples (channel c at
+ * in + c * in_stride, output at out + c * out_stride; strides in floats).
+ * Stream-ordered on a hipStream_t; out must not overlap in. */
+int fftset_blockconv_run(struct fftset_blockconv *bc, float *out, size_t out_stride, const float *in,
+                         size_t in_stride, size_t nsamples, size_t channels, unsigned flags, void *stream);
+
+/* New samples per block (2L - k + 1) and the transform's complex bins. */
+unsigned fftset_blockconv_block_size(const struct fftset_blockconv *bc);
+unsigned fftset_blockconv_complex_bins(const struct fftset_blockconv *bc);
+
+void fftset_blockconv_destroy(struct fftset_blockconv *bc);
+
 /* Plan introspection (for tests, benchmarks and the autotuner). */
 unsigned fftset_fft_complex_bins(const struct fftset_fft *fft);
 /* Writes at most `max` radices of the plan's LDS chain; returns the count
