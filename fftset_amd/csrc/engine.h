@@ -272,13 +272,14 @@ __device__ __forceinline__ void dif_bottom(const Tile& tl, int lt, int T, int s,
 // DIT pass (m = 1: a plain forward DFT) in registers, back to the tile.
 struct BotConv {
     const float2* __restrict__ kern;
-    bool conj_kern;  // true: w <- conj(w * conj(K)) (adjoint w.r.t. the signal)
+    bool conj_kern;      // true: w <- conj(w * conj(K)) (adjoint w.r.t. the signal)
+    bool shared = false;  // one spectrum for every transform: cached loads
     template <int R, class Tile>
     __device__ __forceinline__ void apply(const Tile& tl, int b, int s, float2* v) const
     {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const float2 K = kern[k * s + b];
+            const float2 K = shared ? kern[k * s + b] : ldg(kern + k * s + b);
             v[k] = cconj(conj_kern ? cmulc(v[k], K) : cmul(v[k], K));
         }
         DFT<R>::run(v);
@@ -294,7 +295,7 @@ struct BotKernOut {
     __device__ __forceinline__ void apply(const Tile&, int b, int s, float2* v) const
     {
 #pragma unroll
-        for (int k = 0; k < R; ++k) out[k * s + b] = v[k];
+        for (int k = 0; k < R; ++k) stg(out + k * s + b, v[k]);
     }
 };
 
@@ -307,7 +308,7 @@ struct LdFoMod {
     const float2* __restrict__ tmod;
     __device__ __forceinline__ float2 operator()(int e) const
     {
-        const float re = x[e], im = x[L + e];
+        const float re = ldg(x + e), im = ldg(x + L + e);
         const float2 w = tmod[e];
         return make_float2(re * w.x + im * w.y, re * w.y - im * w.x);
     }
@@ -320,7 +321,7 @@ struct LdFoBins {
     int L;
     __device__ __forceinline__ float2 operator()(int e) const
     {
-        return (e < ((L + 1) >> 1)) ? cconj(in[2 * e]) : in[2 * (L - 1 - e) + 1];
+        return (e < ((L + 1) >> 1)) ? cconj(ldg(in + 2 * e)) : ldg(in + 2 * (L - 1 - e) + 1);
     }
 };
 
@@ -329,7 +330,7 @@ struct LdCx {
     const float2* __restrict__ in;
     __device__ __forceinline__ float2 operator()(int e) const
     {
-        const float2 v = in[e];
+        const float2 v = ldg(in + e);
         return CONJ ? cconj(v) : v;
     }
 };
@@ -342,9 +343,9 @@ struct StFoBins {
     __device__ __forceinline__ void operator()(int f, float2 v) const
     {
         if (f < ((L + 1) >> 1))
-            out[2 * f] = v;
+            stg(out + 2 * f, v);
         else
-            out[2 * (L - 1 - f) + 1] = cconj(v);
+            stg(out + 2 * (L - 1 - f) + 1, cconj(v));
     }
 };
 
@@ -358,15 +359,15 @@ struct StFoDemod {
     __device__ __forceinline__ void operator()(int f, float2 v) const
     {
         const float2 w = tmod[f];
-        y[f] = v.x * w.x - v.y * w.y;
-        y[L + f] = v.x * w.y + v.y * w.x;
+        stg(y + f, v.x * w.x - v.y * w.y);
+        stg(y + L + f, v.x * w.y + v.y * w.x);
     }
 };
 
 template <bool CONJ>
 struct StCx {
     float2* __restrict__ out;
-    __device__ __forceinline__ void operator()(int f, float2 v) const { out[f] = CONJ ? cconj(v) : v; }
+    __device__ __forceinline__ void operator()(int f, float2 v) const { stg(out + f, CONJ ? cconj(v) : v); }
 };
 
 // Writes into the tile (natural order) instead of global memory.

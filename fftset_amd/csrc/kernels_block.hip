@@ -18,7 +18,8 @@
 namespace fsa {
 
 // First pass: frame sample s reads x[s] for lo <= s < n, else 0; then the
-// FREQOFFSETREAL modulation of LdFoMod.
+// FREQOFFSETREAL modulation of LdFoMod. These are cached loads: the k - 1
+// overlap samples of a frame are read again by the next block's frame.
 struct LdFrame {
     const float* __restrict__ x;  // channel base (sample 0)
     long long s0, lo, n;
@@ -44,7 +45,7 @@ struct StFrame {
     {
         const long long o = o0 + p;
         if (p >= skip && o < n)
-            y[o] = v;
+            stg(y + o, v);
     }
     __device__ __forceinline__ void operator()(int f, float2 v) const
     {
@@ -74,7 +75,7 @@ __global__ void __launch_bounds__(1024) k_block(const BlockArgs a)
     const long long s0 = i * a.b - a.skip;
     const LdFrame ld{a.in + c * a.in_st, s0, a.lo, a.n, a.L, a.tmod};
     const StFrame st{a.out + c * a.out_st, s0, a.n, a.L, a.skip, a.tmod};
-    const BotConv bot{(const float2*)a.kern, false};
+    const BotConv bot{(const float2*)a.kern, false, true};
     run_conv<E>(a.c, tl, lt, T, ld, st, bot);
 }
 

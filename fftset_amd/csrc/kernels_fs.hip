@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
                     StCx<true>{(float2*)out}(f, v);
             }
         } else if constexpr (KIND == RK_CONV) {
-            const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st) + (long long)k1c * a.B, a.conj_kern != 0};
+            const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st) + (long long)k1c * a.B, a.conj_kern != 0, a.kern_st == 0};
             run_conv<E>(a.c, tl, lt, T, ld, StRowTw{row_out, a.tfs + (long long)k1c * a.B}, bot);
         } else {
             run_kern<E>(a.c, tl, lt, T, ld, row_out);
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
                     StCx<true>{(float2*)out}(f, v);
             }
         } else if constexpr (KIND == RK_CONV) {
-            const BotConv bot{(const float2*)(a.kern + t * a.kern_st) + (long long)k1 * a.B, a.conj_kern != 0};
+            const BotConv bot{(const float2*)(a.kern + t * a.kern_st) + (long long)k1 * a.B, a.conj_kern != 0, a.kern_st == 0};
             const StRowTw st{row_out, a.tfs + (long long)k1 * a.B};
             if (single) {
                 pf_single_conv<E>(a.c, pf, lt, bot, st);

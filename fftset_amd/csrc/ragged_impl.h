@@ -1,0 +1,193 @@
+// ragged_impl.h -- one launch over a ragged batch of transforms (the kernel;
+// each workgroup size is instantiated in its own kernels_ragged_w*.hip).
+//
+// Every workgroup reads one tile descriptor and runs it with the same LDS
+// engine the uniform kernels use: a tile is either G LDS-resident transforms
+// of one class, or one column / row group of a four-step transform. Classes
+// mix sizes, modulations and operations freely. Tiles are sorted largest
+// first; workgroups are dispatched in blockIdx order, which gives a
+// longest-processing-time-first schedule without a software work queue.
+#pragma once
+
+#include "drivers.h"
+#include "kernels.h"
+
+namespace fsa {
+
+constexpr int kRaggedE = 16;
+constexpr int kIdleLane = 1 << 28;
+
+template <int MOD, int OP>
+__device__ __forceinline__ void ragged_lds(const RClass& k, const RTile& t, float2* smem)
+{
+    const int T = k.T;
+    const int g = threadIdx.x / T;
+    const int lt0 = threadIdx.x - g * T;
+    const bool ok = g < t.count;
+    const int lt = ok ? lt0 : kIdleLane;
+    const int j = t.job + (ok ? g : 0);
+    const TileContig tl{smem + (ok ? g : 0) * lds_len(k.L)};
+    const float* in = k.in[j];
+    float* out = k.out[j];
+    const int L = k.L;
+    if constexpr (OP == OP_FWD) {
+        if constexpr (MOD == MOD_FO)
+            run_stockham<kRaggedE>(k.c, tl, lt, T, LdFoMod{in, L, k.tmod}, StFoBins{(float2*)out, L});
+        else
+            run_stockham<kRaggedE>(k.c, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<false>{(float2*)out});
+    } else if constexpr (OP == OP_INV) {
+        if constexpr (MOD == MOD_FO)
+            run_stockham<kRaggedE>(k.c, tl, lt, T, LdFoBins{(const float2*)in, L}, StFoDemod{out, L, k.tmod});
+        else
+            run_stockham<kRaggedE>(k.c, tl, lt, T, LdCx<true>{(const float2*)in}, StCx<true>{(float2*)out});
+    } else if constexpr (OP == OP_KERN) {
+        if constexpr (MOD == MOD_FO)
+            run_kern<kRaggedE>(k.c, tl, lt, T, LdFoMod{in, L, k.tmod}, (float2*)out);
+        else
+            run_kern<kRaggedE>(k.c, tl, lt, T, LdCx<false>{(const float2*)in}, (float2*)out);
+    } else {
+        const BotConv bot{(const float2*)k.kern[j], OP == OP_GRAD_IN};
+        if constexpr (MOD == MOD_FO)
+            run_conv<kRaggedE>(k.c, tl, lt, T, LdFoMod{in, L, k.tmod}, StFoDemod{out, L, k.tmod}, bot);
+        else
+            run_conv<kRaggedE>(k.c, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<true>{(float2*)out}, bot);
+    }
+}
+
+// Column step of a four-step transform: first (COL) or, for convolutions,
+// the closing step (COL2) that reads the work buffer.
+template <int MOD, int OP, bool SECOND>
+__device__ __forceinline__ void ragged_col(const RClass& k, const RTile& t, float2* smem)
+{
+    const int C = k.colC, T = k.colT;
+    const int g = threadIdx.x % C;
+    const int lt0 = threadIdx.x / C;
+    const int n2 = t.count * C + g;
+    const bool ok = lt0 < T && n2 < k.B;
+    const int lt = ok ? lt0 : kIdleLane;
+    const int n2c = ok ? n2 : 0;
+    const TileInter tl{smem + g, C};
+    const int L = k.L, B = k.B;
+    const int j = t.job;
+    if constexpr (SECOND) {
+        // work -> out, closing the conjugate-data inverse of the convolution
+        const LdCol<LdCx<false>> ld{{(const float2*)k.work[j]}, B, n2c};
+        if constexpr (MOD == MOD_FO)
+            run_stockham<kRaggedE>(k.c, tl, lt, T, ld, StCol<StFoDemod>{{k.out[j], L, k.tmod}, B, n2c});
+        else
+            run_stockham<kRaggedE>(k.c, tl, lt, T, ld, StCol<StCx<true>>{{(float2*)k.out[j]}, B, n2c});
+    } else {
+        // in -> work (get_kernel: in -> out, which it then transforms in place)
+        float2* dst = (float2*)(OP == OP_KERN ? k.out[j] : k.work[j]);
+        const StWorkTw st{dst, B, n2c, k.tfs};
+        const float* in = k.in[j];
+        if constexpr (OP == OP_INV) {
+            if constexpr (MOD == MOD_FO)
+                run_stockham<kRaggedE>(k.c, tl, lt, T, LdCol<LdFoBins>{{(const float2*)in, L}, B, n2c}, st);
+            else
+                run_stockham<kRaggedE>(k.c, tl, lt, T, LdCol<LdCx<true>>{{(const float2*)in}, B, n2c}, st);
+        } else {
+            if constexpr (MOD == MOD_FO)
+                run_stockham<kRaggedE>(k.c, tl, lt, T, LdCol<LdFoMod>{{in, L, k.tmod}, B, n2c}, st);
+            else
+                run_stockham<kRaggedE>(k.c, tl, lt, T, LdCol<LdCx<false>>{{(const float2*)in}, B, n2c}, st);
+        }
+    }
+}
+
+// Row step of a four-step transform.
+template <int MOD, int OP>
+__device__ __forceinline__ void ragged_row(const RClass& k, const RTile& t, float2* smem)
+{
+    const int T = k.rowT, C = k.rowR;
+    const int g = threadIdx.x / T;
+    const int lt0 = threadIdx.x - g * T;
+    const int k1_0 = t.count * C;
+    const int k1 = k1_0 + g;
+    const bool ok = g < C && k1 < k.A;
+    const int lt = ok ? lt0 : kIdleLane;
+    const int k1c = ok ? k1 : 0;
+    const int LS = lds_len(k.B);
+    const TileContig tl{smem + (ok ? g : 0) * LS};
+    const int j = t.job;
+    const float2* src = (const float2*)(OP == OP_KERN ? k.out[j] : k.work[j]);
+    const LdCx<false> ld{src + (long long)k1c * k.B};
+    if constexpr (OP == OP_FWD || OP == OP_INV) {
+        run_stockham_to_tile<kRaggedE>(k.c2, tl, lt, T, ld);
+        __syncthreads();
+        const int nrows = min(C, k.A - k1_0);
+        float* out = k.out[j];
+        const int L = k.L, A = k.A, B = k.B;
+        for (int idx = threadIdx.x; idx < nrows * B; idx += blockDim.x) {
+            const int gg = idx % nrows, k2 = idx / nrows;
+            const float2 v = smem[gg * LS + lds_pad(k2)];
+            const int f = k1_0 + gg + A * k2;
+            if constexpr (OP == OP_FWD) {
+                if constexpr (MOD == MOD_FO)
+                    StFoBins{(float2*)out, L}(f, v);
+                else
+                    StCx<false>{(float2*)out}(f, v);
+            } else {
+                if constexpr (MOD == MOD_FO)
+                    StFoDemod{out, L, k.tmod}(f, v);
+                else
+                    StCx<true>{(float2*)out}(f, v);
+            }
+        }
+    } else if constexpr (OP == OP_KERN) {
+        run_kern<kRaggedE>(k.c2, tl, lt, T, ld, k.out[j] ? (float2*)k.out[j] + (long long)k1c * k.B : nullptr);
+    } else {
+        float2* row = (float2*)k.work[j] + (long long)k1c * k.B;
+        const BotConv bot{(const float2*)k.kern[j] + (long long)k1c * k.B, OP == OP_GRAD_IN};
+        run_conv<kRaggedE>(k.c2, tl, lt, T, ld, StRowTw{row, k.tfs + (long long)k1c * k.B}, bot);
+    }
+}
+
+template <int MOD, int OP>
+__device__ __forceinline__ void ragged_dispatch_kind(const RClass& k, const RTile& t, float2* smem)
+{
+    switch (t.kind) {
+    case TK_LDS: ragged_lds<MOD, OP>(k, t, smem); break;
+    case TK_COL: ragged_col<MOD, OP, false>(k, t, smem); break;
+    case TK_ROW: ragged_row<MOD, OP>(k, t, smem); break;
+    case TK_COL2:
+        if constexpr (OP == OP_CONV || OP == OP_GRAD_IN)
+            ragged_col<MOD, OP, true>(k, t, smem);
+        break;
+    default: break;
+    }
+}
+
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
+{
+    extern __shared__ float2 smem[];
+    const RTile t = a.tiles[blockIdx.x];
+    const RClass& k = a.classes[t.cls];
+    const int sel = k.mod * OP_COUNT + k.op;
+    switch (sel) {
+    case MOD_FO * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_FO, OP_FWD>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_FO, OP_INV>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_FO, OP_KERN>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_FO, OP_CONV>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_FO, OP_GRAD_IN>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_CX, OP_FWD>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_CX, OP_INV>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_CX, OP_KERN>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_CX, OP_CONV>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_CX, OP_GRAD_IN>(k, t, smem); break;
+    default: break;
+    }
+}
+
+template <int THREADS>
+hipError_t launch_w(const RaggedArgs& a, int lds_bytes, hipStream_t st)
+{
+    if (lds_bytes > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_ragged<THREADS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  lds_bytes);
+    hipLaunchKernelGGL(k_ragged<THREADS>, dim3(a.ntiles), dim3(THREADS), lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace fsa
