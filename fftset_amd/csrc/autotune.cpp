@@ -4,7 +4,7 @@
 // (src/fftset_vec.c:1004-1062). Here the candidates are timed on the device:
 // for a size that fits LDS, register-radix chains (candidate_chains) crossed
 // with transforms-per-workgroup packings; for a larger size, the four-step
-// splits closest to balanced, each with 16, 32 or 64 columns per column tile
+// splits up to 16:1, each with 16, 32 or 64 columns per column tile
 // (stored in the G field of the table). The score is the time of the
 // requested operations (forward and/or convolution, one launch each) over a
 // batch of about 512 MiB (large enough that launch overhead and clock
@@ -215,12 +215,15 @@ int fftset_autotune(struct fftset* fc, const struct fftset_modulation* modulatio
     }
     if (cands.empty()) {
         std::vector<std::pair<double, unsigned>> splits;
-        for (unsigned A = 2; A * A <= complex_bins; ++A)
+        for (unsigned A = 2; A < complex_bins; ++A)
             if (complex_bins % A == 0 && fits_lds(A) && fits_lds(complex_bins / A))
                 splits.push_back({std::abs(std::log2((double)(complex_bins / A)) - std::log2((double)A)), A});
         std::sort(splits.begin(), splits.end());
-        for (size_t i = 0; i < splits.size() && i < 4; ++i)
-            for (int C : {16, 32, 64}) cands.push_back({{}, C, splits[i].second});
+        // every split up to 16:1 either way (short columns allow wide,
+        // better-coalesced column tiles), each with three tile widths
+        for (size_t i = 0; i < splits.size(); ++i)
+            if (splits[i].first <= 4.0)
+                for (int C : {16, 32, 64}) cands.push_back({{}, C, splits[i].second});
     }
 
     // buffers: about 512 MiB of transforms

@@ -402,7 +402,7 @@ struct StWorkTw {
     const float2* __restrict__ tfs;
     __device__ __forceinline__ void operator()(int k1, float2 v) const
     {
-        work[k1 * B + n2] = cmul(v, tfs[k1 * B + n2]);
+        stg(work + k1 * B + n2, cmul(v, tfs[k1 * B + n2]));
     }
 };
 
@@ -411,7 +411,7 @@ struct StWorkTw {
 struct StRowTw {
     float2* __restrict__ row;
     const float2* __restrict__ trow;
-    __device__ __forceinline__ void operator()(int n2, float2 v) const { row[n2] = cmul(v, trow[n2]); }
+    __device__ __forceinline__ void operator()(int n2, float2 v) const { stg(row + n2, cmul(v, trow[n2])); }
 };
 
 }  // namespace fsa
