@@ -46,8 +46,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="ragged: launch phases directly")
     ap.add_argument("--sched", default="branches", choices=["branches", "phased", "serial"],
                     help="ragged: order of the tile groups inside the graph")
-    ap.add_argument("--baselines", default="group,transform",
-                    help="ragged: comma list of baselines to time (group, transform, none)")
+    ap.add_argument("--baselines", default="group,pass,transform",
+                    help="ragged: comma list of baselines to time (group, pass, transform, none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
@@ -329,9 +329,10 @@ def workload_ragged(torch, fs, rng_np, args):
 
     def baselines(torch, steps):
         out = {}
-        modes = [m for m in args.baselines.split(",") if m in ("group", "transform")]
+        modes = [m for m in args.baselines.split(",") if m in ("group", "pass", "transform")]
         for m in modes:
-            mode = fftset_amd.BASELINE_PER_GROUP if m == "group" else fftset_amd.BASELINE_PER_TRANSFORM
+            mode = {"group": fftset_amd.BASELINE_PER_GROUP, "pass": fftset_amd.BASELINE_PER_PASS,
+                    "transform": fftset_amd.BASELINE_PER_TRANSFORM}[m]
             reps = steps if m == "group" else max(1, min(3, steps))
             b.launch_baseline(mode)
             torch.cuda.synchronize()

@@ -236,6 +236,7 @@ BATCH_SCHED_SERIAL = 4
 BATCH_SCHED_PHASED = 8
 BASELINE_PER_GROUP = 1
 BASELINE_PER_TRANSFORM = 2
+BASELINE_PER_PASS = 3
 BLOCKCONV_HISTORY = 1
 
 

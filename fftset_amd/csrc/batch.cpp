@@ -40,6 +40,8 @@ struct BatchClass {
     FourStep fs;
     int threads;            // workgroup size of this class's LDS tiles
     int col_threads, row_threads;  // four-step: column and row tiles
+    std::vector<int> group_idx;    // this class's entries of group_ranges
+    long long pass_off = 0;        // per-pass baseline: offset of its work tiles (float2)
 };
 
 // A contiguous run of tiles launched together with one workgroup size.
@@ -72,6 +74,7 @@ struct fftset_batch {
     int sched = 0;                   // FFTSET_BATCH_SCHED_*
     std::vector<TileRange> group_ranges;
     std::vector<TileRange> job_ranges;
+    float2* pass_work = nullptr;     // per-pass baseline work tiles (allocated on first use)
     int lds_bytes = 0;  // largest of any range
     long long alg_bytes = 0;
     hipGraph_t graph = nullptr;
@@ -268,6 +271,7 @@ static int commit(fftset_batch* b, hipStream_t st)
             if (g3[q].empty())
                 continue;
             const int kind = g3[q][0].kind;
+            b->classes[ci].group_idx.push_back((int)b->group_ranges.size());
             b->group_ranges.push_back(
                 {(int)group.size(), (int)g3[q].size(), tile_threads(k, kind), tile_lds_bytes(k, kind)});
             group.insert(group.end(), g3[q].begin(), g3[q].end());
@@ -567,6 +571,35 @@ int fftset_batch_launch_baseline(struct fftset_batch* b, int mode, void* stream)
         for (const TileRange& r : b->group_ranges)
             if ((rc = launch_range(b, b->d_group_tiles, r, st)) != FFTSET_OK)
                 break;
+    } else if (mode == FFTSET_BASELINE_PER_PASS) {
+        if (!b->pass_work) {
+            long long n = 0;
+            for (auto& k : b->classes)
+                if (!k.four_step) {
+                    k.pass_off = n;
+                    n += (long long)k.rc.L * (long long)k.jobs.size();
+                }
+            if (n > 0 && hipMalloc((void**)&b->pass_work, (size_t)n * sizeof(float2)) != hipSuccess) {
+                b->pass_work = nullptr;
+                set_error(FFTSET_E_NOMEM);
+                return FFTSET_E_NOMEM;
+            }
+        }
+        for (size_t ci = 0; ci < b->classes.size() && rc == FFTSET_OK; ++ci) {
+            const BatchClass& k = b->classes[ci];
+            if (k.four_step) {
+                for (int g : k.group_idx)
+                    if ((rc = launch_range(b, b->d_group_tiles, b->group_ranges[g], st)) != FFTSET_OK)
+                        break;
+                continue;
+            }
+            const int steps = pass_steps(k.op, k.rc.c.np);
+            for (int s = 0; s < steps && rc == FFTSET_OK; ++s)
+                rc = launch_pass(b->d_classes + ci, s, (int)k.jobs.size(), k.rc.T, b->pass_work + k.pass_off, st) ==
+                             hipSuccess
+                         ? FFTSET_OK
+                         : FFTSET_E_DEVICE;
+        }
     } else if (mode == FFTSET_BASELINE_PER_TRANSFORM) {
         // one launch per transform and step (job_ranges holds 3 per transform)
         for (const TileRange& r : b->job_ranges)
@@ -623,12 +656,14 @@ void fftset_batch_destroy(struct fftset_batch* b)
     }
     if (b->fork)
         (void)hipEventDestroy(b->fork);
-    if (b->dev_mem || b->work_pool)
+    if (b->dev_mem || b->work_pool || b->pass_work)
         (void)hipDeviceSynchronize();
     if (b->dev_mem)
         (void)hipFree(b->dev_mem);
     if (b->work_pool)
         (void)hipFree(b->work_pool);
+    if (b->pass_work)
+        (void)hipFree(b->pass_work);
     delete b;
 }
 

@@ -83,6 +83,11 @@ struct RaggedArgs {
 // Launches tiles [0, ntiles) of `a`; every class must fit `threads` lanes.
 hipError_t launch_ragged(const RaggedArgs& a, int threads, int lds_bytes, hipStream_t st);
 
+// Per-pass baseline (kernels_pass.hip): step `step` of every transform of
+// one class, one workgroup per transform, data in `work` between steps.
+int pass_steps(int op, int np);
+hipError_t launch_pass(const RClass* d_cls, int step, int ntrans, int T, float2* work, hipStream_t st);
+
 // ------------------------------------------------------- block convolution
 // Overlap-save FIR filtering of long real signals (see kernels_block.hip).
 struct BlockArgs {

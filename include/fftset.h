@@ -163,6 +163,9 @@ struct fftset_batch;
 
 #define FFTSET_BASELINE_PER_GROUP      1  /* one launch per (plan, operation) group and step */
 #define FFTSET_BASELINE_PER_TRANSFORM  2  /* one launch per transform and step               */
+#define FFTSET_BASELINE_PER_PASS       3  /* the reference's pass structure: one launch per radix
+                                             pass of each group, HBM round trip between passes
+                                             (four-step groups: one launch per step)           */
 
 /* Creates an empty batch on the set; NULL on failure. */
 struct fftset_batch *fftset_batch_create(struct fftset *fc);

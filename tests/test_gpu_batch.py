@@ -129,6 +129,29 @@ def test_graph_replays_and_baselines_reproduce_the_launch(fs):
     b.destroy()
 
 
+def test_per_pass_baseline_matches_the_fused_launch(fs, oracle):
+    # the reference's pass structure (one launch per radix pass, HBM between
+    # passes) computes the same transforms as the fused LDS kernels
+    rng = np.random.default_rng(9)
+    jobs = _make_jobs(rng, SPEC)
+    b = _build(fs, jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    first = [jb["dy"].cpu().numpy().copy() for jb in jobs]
+    for rep in range(2):
+        for jb in jobs:
+            jb["dy"].fill_(float("nan"))
+        b.launch_baseline(fftset_amd.BASELINE_PER_PASS)
+        torch.cuda.synchronize()
+        for jb, fused in zip(jobs, first):
+            y = jb["dy"].cpu().numpy()
+            assert np.all(np.isfinite(y)), (jb["mod"], jb["L"], jb["op"])
+            assert fp64.rel_rms(y, fused) <= 1e-6, (jb["mod"], jb["L"], jb["op"])
+            ref = _oracle_ref(oracle, jb)
+            assert fp64.rel_rms(y, ref) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
+    b.destroy()
+
+
 def test_batch_rejects_bad_use(fs):
     b = fs.batch()
     f = fs.create_fft(CX, 64)
