@@ -180,3 +180,30 @@ def test_plan_cache_returns_same_plan(fs):
     b = lib.fftset_create_fft(fs.address, fftset_amd._modulation(FO), 96)
     c = lib.fftset_create_fft(fs.address, fftset_amd._modulation(CX), 96)
     assert a and a == b and c != a
+
+
+# ------------------------------------------------ sizes beyond 65536 bins
+LARGE = [131072, 245760, 1048576, 4194304]
+
+
+@pytest.mark.parametrize("mod", [FO, CX])
+@pytest.mark.parametrize("L", LARGE)
+def test_large_sizes_fp64(fs, oracle, mod, L):
+    # four-step with both factors LDS-resident (up to 16384 x 16384 bins)
+    rng = np.random.default_rng(3000 + L)
+    f = fs.create_fft(mod, L)
+    radices = f.radices()
+    assert 0 in radices  # column chain, 0, row chain
+    x = rng.standard_normal(2 * L).astype(np.float32)
+    h = (rng.standard_normal(2 * L) * np.exp(-np.arange(2 * L) / (L / 4))).astype(np.float32)
+    y = gpu_forward(f, x)
+    c = gpu_conv(f, x, h)
+    if mod == FO:
+        assert fp64.rel_rms(y, fp64.fo_forward64(x)) <= fp64_tol(L)
+        assert fp64.rel_rms(c, fp64.fo_conv64(x, h)) <= fp64_tol(L)
+    else:
+        assert fp64.rel_rms(y, fp64.cx_forward64(x)) <= fp64_tol(L)
+        assert fp64.rel_rms(c, fp64.cx_conv64(x, h)) <= fp64_tol(L)
+    if L <= 1048576:
+        o = oracle.plan(OFO if mod == FO else OCX, L)
+        assert fp64.rel_rms(y, o.forward(x)) <= REF_TOL
