@@ -412,6 +412,59 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
     return FFTSET_E_INVALID;
 }
 
+// True for memory the device cannot address as its own: pageable or pinned
+// host memory. The drop-in calls accept such buffers, as the reference's
+// callers pass, and stage them through device memory.
+static bool is_host_ptr(const void* ptr)
+{
+    if (!ptr)
+        return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+        (void)hipGetLastError();  // unknown to the runtime: plain host memory
+        return true;
+    }
+    return a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged;
+}
+
+// One drop-in call with some buffers in host memory: copies them through the
+// set's staging buffer (4 x 2L floats) around the same execution.
+static int staged_call(const fftset_fft* p, int op, float* out, const float* in, const float* kern, float* work,
+                       hipStream_t st)
+{
+    Ctx* ctx = p->ctx;
+    const long long len2 = 2LL * p->lfft;
+    const size_t bytes = (size_t)len2 * sizeof(float);
+    std::lock_guard<std::mutex> g(ctx->stage_mu);
+    if (ctx->stage_floats < (size_t)(4 * len2)) {
+        if (ctx->stage) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(ctx->stage);
+        }
+        ctx->stage = nullptr;
+        ctx->stage_floats = 0;
+        if (hipMalloc((void**)&ctx->stage, (size_t)(4 * len2) * sizeof(float)) != hipSuccess)
+            return FFTSET_E_NOMEM;
+        ctx->stage_floats = (size_t)(4 * len2);
+    }
+    float* s = ctx->stage;
+    const bool h_out = is_host_ptr(out), h_in = is_host_ptr(in), h_k = is_host_ptr(kern), h_w = is_host_ptr(work);
+    float* d_out = h_out ? s : out;
+    const float* d_in = h_in ? s + len2 : in;
+    const float* d_k = h_k ? s + 2 * len2 : kern;
+    float* d_w = h_w ? s + 3 * len2 : work;
+    if (h_in && hipMemcpyAsync((void*)d_in, in, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+        return FFTSET_E_DEVICE;
+    if (h_k && hipMemcpyAsync((void*)d_k, kern, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+        return FFTSET_E_DEVICE;
+    int rc = execute(p, op, 1, d_out, len2, d_in, len2, d_k, len2, d_w, len2, st);
+    if (rc == FFTSET_OK && h_out && hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = FFTSET_E_DEVICE;
+    if (hipStreamSynchronize(st) != hipSuccess && rc == FFTSET_OK)
+        rc = FFTSET_E_DEVICE;
+    return rc;
+}
+
 static void sync_call(const fftset_fft* p, int op, float* out, const float* in, const float* kern, float* work)
 {
     if (!p) {
@@ -426,6 +479,10 @@ static void sync_call(const fftset_fft* p, int op, float* out, const float* in, 
     hipStream_t st = thread_stream(p->ctx->device);
     if (!st) {
         set_error(FFTSET_E_DEVICE);
+        return;
+    }
+    if (is_host_ptr(out) || is_host_ptr(in) || is_host_ptr(kern) || is_host_ptr(work)) {
+        set_error(staged_call(p, op, out, in, kern, work, st));
         return;
     }
     const long long len2 = 2LL * p->lfft;
@@ -475,6 +532,8 @@ void fftset_destroy(struct fftset* fc)
         (void)hipFree(t.dev);
     if (ctx->scratch)
         (void)hipFree(ctx->scratch);
+    if (ctx->stage)
+        (void)hipFree(ctx->stage);
     delete ctx;
     fc->impl = nullptr;
 }

@@ -59,6 +59,9 @@ struct Ctx {
     std::mutex mu;                // guards scratch
     float* scratch = nullptr;     // library-owned work space for batched calls
     size_t scratch_floats = 0;
+    std::mutex stage_mu;          // guards stage
+    float* stage = nullptr;       // device copies of host buffers passed to the drop-in calls
+    size_t stage_floats = 0;
 };
 
 enum Engine : int { ENG_LDS = 0, ENG_FOUR_STEP = 1 };

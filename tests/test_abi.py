@@ -2,6 +2,7 @@
 public header declares, and its host-only helper matches the oracle."""
 import ctypes
 import re
+import subprocess
 
 import pytest
 
@@ -53,3 +54,12 @@ def test_null_arguments_are_rejected_without_a_device():
     lib.fftset_fft_forward(None, None, None, None)
     assert lib.fftset_last_error() == fftset_amd.E_INVALID
     assert lib.fftset_last_error() == fftset_amd.OK
+
+
+def test_header_and_library_link_from_plain_c(tmp_path):
+    # the drop-in header is C (not C++) and the library links into a C program
+    import __graft_entry__ as ge
+    out = tmp_path / "dropin_host"
+    r = subprocess.run(ge.c_dropin_command(str(out)), capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert out.exists()
