@@ -35,6 +35,14 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "ragged-batch FFT/conv GB/s & %HBM peak vs per-size launches at ≤2e-6 rel RMS"
 
 
+def nominal_flop(L, op):
+    """5 L log2 L per transform of L complex bins; a convolution (and its
+    adjoints) counts two transforms and the 6 L flops of the product."""
+    import math
+    t = 5.0 * L * math.log2(L) if L > 1 else 0.0
+    return 2 * t + 6.0 * L if op == "conv" else t
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +104,7 @@ def workload_c2(torch, fs, rng_np, args):
     }
     bytes_per = {"forward": 16 * L * B, "inverse": 16 * L * B}
     cfg = {"workload": "uniform COMPLEX forward+inverse", "complex_bins": L, "batch": B,
-           "radices": f.radices()}
+           "radices": f.radices(), "transforms_per_step": 2 * B, "nominal_flop_per_step": 2 * B * nominal_flop(L, "fwd")}
 
     def check():
         from oracle import Oracle, fp64
@@ -129,7 +137,8 @@ def workload_c3(torch, fs, rng_np, args):
     kernels = {"conv": lambda: f.execute(fftset_amd.OP_CONV, B, y, x, kern=K)}
     bytes_per = {"conv": 24 * L * B}
     cfg = {"workload": "multichannel FREQOFFSETREAL convolution (kernel spectra precomputed)",
-           "complex_bins": L, "channels": B, "kernel_taps": 4096, "radices": f.radices()}
+           "complex_bins": L, "channels": B, "kernel_taps": 4096, "radices": f.radices(),
+           "transforms_per_step": B, "nominal_flop_per_step": B * nominal_flop(L, "conv")}
 
     def check():
         from oracle import Oracle, fp64
@@ -165,7 +174,9 @@ def workload_c5(torch, fs, rng_np, args):
     bytes_per = {"get_kernel": 16 * L * B, "conv": 24 * L * B, "grad_input": 24 * L * B,
                  "grad_kernel": 24 * L * B}
     cfg = {"workload": "differentiable FREQOFFSETREAL convolution (forward + both adjoints)",
-           "complex_bins": L, "channels": B, "radices": f.radices()}
+           "complex_bins": L, "channels": B, "radices": f.radices(), "transforms_per_step": 4 * B,
+           # get_kernel + conv + grad_input + grad_kernel (a forward and a grad_input)
+           "nominal_flop_per_step": B * (2 * nominal_flop(L, "fwd") + 3 * nominal_flop(L, "conv"))}
 
     def check():
         from oracle import fp64
@@ -191,8 +202,10 @@ def workload_block(torch, fs, rng_np, args):
     bc = fs.blockconv(h, 0)
     kernels = {"block_conv": lambda: bc.run(y, x)}
     bytes_per = {"block_conv": 8 * C * n}
+    frames = C * ((n + bc.block_size - 1) // bc.block_size)
     cfg = {"workload": "overlap-save FIR filter of long real signals", "taps": k, "channels": C,
-           "samples_per_channel": n, "complex_bins": bc.complex_bins, "block_size": bc.block_size}
+           "samples_per_channel": n, "complex_bins": bc.complex_bins, "block_size": bc.block_size,
+           "transforms_per_step": frames, "nominal_flop_per_step": frames * nominal_flop(bc.complex_bins, "conv")}
 
     def check():
         from oracle import fp64
@@ -297,31 +310,45 @@ def workload_ragged(torch, fs, rng_np, args):
            "transforms": n, "sizes": RAGGED_SIZES, "conv_fraction": round(float(is_conv.mean()), 4),
            "algorithmic_bytes": b.bytes, "tiles": info["tiles"], "classes": info["classes"],
            "four_step_classes": info["four_step_classes"], "lds_bytes": info["lds_bytes"],
-           "threads": args.threads, "graph": not args.no_graph, "schedule": args.sched}
+           "threads": args.threads, "graph": not args.no_graph, "schedule": args.sched,
+           "transforms_per_step": n,
+           "nominal_flop_per_step": float(sum(nominal_flop(int(L), "conv" if c else "fwd")
+                                              for L, c in zip(sizes, is_conv)))}
 
     def check():
         from oracle import Oracle, fp64
         from oracle.oracle import MOD_COMPLEX, MOD_FREQ_OFFSET_REAL
         orc = Oracle()
         worst = 0.0
-        per_size = {}
+        vs_orc, vs_64 = {}, {}
+        fp64_ok = True
         for L in RAGGED_SIZES:
             for conv in (False, True):
-                idx = np.nonzero((sizes == L) & (is_conv == conv))[0][:2]
+                idx = np.nonzero((sizes == L) & (is_conv == conv))[0][:4]
+                key = f"{'conv' if conv else 'fwd'}{L}"
                 for i in idx:
                     a, e = int(off[i]), int(off[i + 1])
                     xs = x[a:e].cpu().numpy()
                     ys = y[a:e].cpu().numpy()
                     if conv:
+                        hs = hk[a:e].cpu().numpy()
                         o = orc.plan(MOD_FREQ_OFFSET_REAL, L)
-                        ref = o.conv(xs, o.get_kernel(hk[a:e].cpu().numpy()))
+                        ref = o.conv(xs, o.get_kernel(hs))
+                        ref64 = fp64.fo_conv64(xs, hs)
                     else:
                         ref = orc.plan(MOD_COMPLEX, L).forward(xs)
+                        ref64 = fp64.cx_forward64(xs)
                     err = float(fp64.rel_rms(ys, ref))
-                    key = f"{'conv' if conv else 'fwd'}{L}"
-                    per_size[key] = max(per_size.get(key, 0.0), err)
+                    e64 = float(fp64.rel_rms(ys, ref64))
+                    vs_orc.setdefault(key, []).append(err)
+                    vs_64.setdefault(key, []).append(e64)
+                    fp64_ok &= e64 <= 1e-6 * np.log2(L)
                     worst = max(worst, err)
-        cfg["rel_rms_vs_oracle_by_class"] = {k: float(f"{v:.3g}") for k, v in per_size.items()}
+        fmt = lambda d: {k: {"max": float(f"{max(v):.3g}"), "median": float(f"{np.median(v):.3g}")}
+                         for k, v in d.items()}
+        cfg["rel_rms_vs_oracle_by_class"] = fmt(vs_orc)
+        cfg["rel_rms_vs_fp64_by_class"] = fmt(vs_64)
+        cfg["fp64_within_1e-6_log2L"] = bool(fp64_ok)
         return worst
 
     def cpu_sample(seconds):
@@ -501,6 +528,10 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
         "tune_file": os.path.relpath(tune, ROOT) if tune else None,
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "transforms_per_s": round(cfg["transforms_per_step"] / (step_ms * 1e-3), 1)
+        if "transforms_per_step" in cfg else None,
+        "nominal_gflop_per_s": round(cfg["nominal_flop_per_step"] / (step_ms * 1e-3) / 1e9, 1)
+        if "nominal_flop_per_step" in cfg else None,
         "max_rel_rms_vs_oracle": err,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
