@@ -67,6 +67,17 @@ def _oracle_ref(oracle, jb):
         return o.inverse(x)
     if op == fftset_amd.OP_CONV:
         return o.conv(x, o.get_kernel(jb["h"]))
+    if op == fftset_amd.OP_CONV_GRAD_INPUT:
+        # adjoint w.r.t. the signal: L * negacyclic (FREQOFFSETREAL) or L * cyclic (COMPLEX)
+        # correlation with h, checked in FP64
+        L = jb["L"]
+        if jb["mod"] == FO:
+            return fp64.fo_conv_grads64(x[None], np.zeros_like(x)[None], jb["h"][None])[0][0]
+        X = np.fft.fft(fp64.as_complex(x))
+        H = np.fft.fft(fp64.as_complex(jb["h"]))
+        return fp64.as_interleaved(np.fft.ifft(X * np.conj(H)) * L)
+    if op == fftset_amd.OP_GET_KERNEL:
+        return None
     return None
 
 
@@ -80,6 +91,8 @@ SPEC = [
     (CX, 15360, fftset_amd.OP_FORWARD, 3), (FO, 15360, fftset_amd.OP_CONV, 3),
     (CX, 61440, fftset_amd.OP_FORWARD, 2), (FO, 61440, fftset_amd.OP_CONV, 2),
     (FO, 61440, fftset_amd.OP_INVERSE, 1), (CX, 61440, fftset_amd.OP_CONV, 1),
+    (FO, 768, fftset_amd.OP_CONV_GRAD_INPUT, 3), (CX, 360, fftset_amd.OP_CONV_GRAD_INPUT, 3),
+    (FO, 61440, fftset_amd.OP_CONV_GRAD_INPUT, 1), (CX, 120, fftset_amd.OP_GET_KERNEL, 2),
 ]
 
 
@@ -94,10 +107,17 @@ def test_ragged_batch_matches_oracle(fs, oracle, flags):
     info = b.info()
     assert info["classes"] == len({(m, L, op) for m, L, op, _ in SPEC})
     for jb in jobs:
-        ref = _oracle_ref(oracle, jb)
         y = jb["dy"].cpu().numpy()
         assert np.all(np.isfinite(y)), (jb["mod"], jb["L"], jb["op"])
-        assert fp64.rel_rms(y, ref) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
+        if jb["op"] == fftset_amd.OP_GET_KERNEL:
+            # opaque spectrum: equal to the drop-in call on the same plan
+            k = torch.empty_like(jb["dx"])
+            jb["f"].conv_get_kernel(k, jb["dx"])
+            assert fp64.rel_rms(y, k.cpu().numpy()) <= 1e-6
+            continue
+        ref = _oracle_ref(oracle, jb)
+        tol = REF_TOL if jb["op"] != fftset_amd.OP_CONV_GRAD_INPUT else 1e-6 * np.log2(jb["L"])
+        assert fp64.rel_rms(y, ref) <= tol, (jb["mod"], jb["L"], jb["op"])
     b.destroy()
 
 
@@ -147,8 +167,8 @@ def test_per_pass_baseline_matches_the_fused_launch(fs, oracle):
             y = jb["dy"].cpu().numpy()
             assert np.all(np.isfinite(y)), (jb["mod"], jb["L"], jb["op"])
             assert fp64.rel_rms(y, fused) <= 1e-6, (jb["mod"], jb["L"], jb["op"])
-            ref = _oracle_ref(oracle, jb)
-            assert fp64.rel_rms(y, ref) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
+            if jb["op"] in (fftset_amd.OP_FORWARD, fftset_amd.OP_INVERSE, fftset_amd.OP_CONV):
+                assert fp64.rel_rms(y, _oracle_ref(oracle, jb)) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
     b.destroy()
 
 
