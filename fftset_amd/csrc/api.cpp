@@ -94,21 +94,31 @@ bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st)
     return true;
 }
 
-static int ensure_scratch(Ctx* ctx, size_t floats, float** out)
+// Work space of at least `floats` for calls on stream `st`; a buffer that
+// grows is freed only after the stream's earlier work has finished.
+static int ensure_scratch(Ctx* ctx, hipStream_t st, size_t floats, float** out)
 {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    if (ctx->scratch_floats < floats) {
-        if (ctx->scratch) {
-            (void)hipDeviceSynchronize();
-            (void)hipFree(ctx->scratch);
-            ctx->scratch = nullptr;
-            ctx->scratch_floats = 0;
-        }
-        if (hipMalloc((void**)&ctx->scratch, floats * sizeof(float)) != hipSuccess)
-            return FFTSET_E_NOMEM;
-        ctx->scratch_floats = floats;
+    Ctx::Scratch* s = nullptr;
+    for (Ctx::Scratch& c : ctx->scratch)
+        if (c.stream == st)
+            s = &c;
+    if (!s) {
+        ctx->scratch.push_back({st, nullptr, 0});
+        s = &ctx->scratch.back();
     }
-    *out = ctx->scratch;
+    if (s->floats < floats) {
+        if (s->buf) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(s->buf);
+            s->buf = nullptr;
+            s->floats = 0;
+        }
+        if (hipMalloc((void**)&s->buf, floats * sizeof(float)) != hipSuccess)
+            return FFTSET_E_NOMEM;
+        s->floats = floats;
+    }
+    *out = s->buf;
     return FFTSET_OK;
 }
 
@@ -344,7 +354,7 @@ static int run_op(const fftset_fft* p, int op, long long count, float* out, long
         return run_lds(p, op, count, out, out_st, in, in_st, kern, kern_st, st);
     const long long len2 = 2LL * p->lfft;
     if (op != OP_KERN && !work) {
-        int rc = ensure_scratch(p->ctx, (size_t)(count * len2), &work);
+        int rc = ensure_scratch(p->ctx, st, (size_t)(count * len2), &work);
         if (rc != FFTSET_OK)
             return rc;
         work_st = len2;
@@ -393,7 +403,7 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
         if (!w || four) {
             const size_t need = (size_t)(count * len2) * ((!w && four) ? 2 : 1);
             float* s = nullptr;
-            int rc = ensure_scratch(p->ctx, need, &s);
+            int rc = ensure_scratch(p->ctx, st, need, &s);
             if (rc != FFTSET_OK)
                 return rc;
             if (!w) {
@@ -530,8 +540,9 @@ void fftset_destroy(struct fftset* fc)
     }
     for (Table& t : ctx->tables)
         (void)hipFree(t.dev);
-    if (ctx->scratch)
-        (void)hipFree(ctx->scratch);
+    for (Ctx::Scratch& c : ctx->scratch)
+        if (c.buf)
+            (void)hipFree(c.buf);
     if (ctx->stage)
         (void)hipFree(ctx->stage);
     delete ctx;

@@ -57,8 +57,15 @@ struct Ctx {
     size_t table_bytes = 0;
     fftset_fft* first = nullptr;  // plans sorted by descending size (src/fftset.c:97-129)
     std::mutex mu;                // guards scratch
-    float* scratch = nullptr;     // library-owned work space for batched calls
-    size_t scratch_floats = 0;
+    // Library-owned work space for calls that pass no work buffer, one per
+    // stream: calls on one stream are ordered, calls on different streams
+    // never share a buffer.
+    struct Scratch {
+        hipStream_t stream;
+        float* buf;
+        size_t floats;
+    };
+    std::vector<Scratch> scratch;
     std::mutex stage_mu;          // guards stage
     float* stage = nullptr;       // device copies of host buffers passed to the drop-in calls
     size_t stage_floats = 0;

@@ -130,7 +130,8 @@ int fftset_last_error(void);
  *   CONV_GRAD_KERNEL: out = adjoint w.r.t. the kernel taps applied to kern
  *                     (the output gradient) with in = the signal x
  * work (may be NULL) provides count*work_stride floats of scratch for the
- * operations that need it; when NULL the library uses its own. stream is a
+ * operations that need it; when NULL the library uses its own, one buffer
+ * per stream (so calls on different streams never share it). stream is a
  * hipStream_t (NULL = the default stream). Returns FFTSET_OK or an error. */
 int fftset_fft_execute(const struct fftset_fft *fft, int op, size_t count, float *out, size_t out_stride,
                        const float *in, size_t in_stride, const float *kern, size_t kern_stride, float *work,

@@ -207,3 +207,30 @@ def test_large_sizes_fp64(fs, oracle, mod, L):
     if L <= 1048576:
         o = oracle.plan(OFO if mod == FO else OCX, L)
         assert fp64.rel_rms(y, o.forward(x)) <= REF_TOL
+
+
+def test_concurrent_streams_without_work_buffers(fs):
+    # four-step calls that pass no work buffer use library scratch; calls on
+    # two streams at once must not share it
+    import torch
+    L, n = 61440, 24
+    f = fs.create_fft(FO, L)
+    rng = np.random.default_rng(77)
+    xs = [rng.standard_normal((n, 2 * L)).astype(np.float32) for _ in range(2)]
+    hs = [(rng.standard_normal((n, 2 * L)) * np.exp(-np.arange(2 * L) / (L / 4))).astype(np.float32)
+          for _ in range(2)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = []
+    for x, h, s in zip(xs, hs, streams):
+        with torch.cuda.stream(s):
+            dx, dh = torch.from_numpy(x).cuda(), torch.from_numpy(h).cuda()
+            k = torch.empty_like(dx)
+            y = torch.empty_like(dx)
+            f.execute(fftset_amd.OP_GET_KERNEL, n, k, dh, stream=s)
+            for _ in range(3):  # keep both streams busy at the same time
+                f.execute(fftset_amd.OP_CONV, n, y, dx, kern=k, stream=s)
+            outs.append((y, dx, dh, k))
+    torch.cuda.synchronize()
+    for (y, _, _, _), x, h in zip(outs, xs, hs):
+        ref = fp64.fo_conv64(x[[0, n - 1]], h[[0, n - 1]])
+        assert np.max(fp64.rel_rms(y.cpu().numpy()[[0, n - 1]], ref)) <= fp64_tol(L)
