@@ -90,6 +90,7 @@ hipError_t launch_block(const BlockArgs& a, const ShapeD& sh, hipStream_t st)
         return hipErrorInvalidValue;
     if (sh.lds_bytes > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_block<16>, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
+    (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(k_block<16>, dim3((unsigned)tiles), dim3(sh.threads), sh.lds_bytes, st, a);
     return hipGetLastError();
 }
@@ -104,6 +105,7 @@ __global__ void k_taps(float* __restrict__ dst, const float* __restrict__ taps, 
 hipError_t launch_taps(float* dst, const float* taps, int k, int n, float scale, hipStream_t st)
 {
     const int grid = std::min(1024, (n + 255) / 256);
+    (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(k_taps, dim3(grid), dim3(256), 0, st, dst, taps, k, n, scale);
     return hipGetLastError();
 }

@@ -266,6 +266,7 @@ static hipError_t do_launch(F fn, F fn_persist, const FsArgs& a, const ShapeD& s
     if (sh.lds_bytes > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
     const long long grid = persist ? persistent_grid((const void*)fn, sh.threads, sh.lds_bytes, tiles) : tiles;
+    (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(sh.threads), sh.lds_bytes, st, a);
     return hipGetLastError();
 }

@@ -38,6 +38,7 @@ hipError_t launch_twiddles(float2* out, long long N, int a0, int rows, int cols,
 {
     const long long n = (long long)rows * cols;
     const int threads = 256;
+    (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(k_twiddle_fp64, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads), 0, st, out, N,
                        a0, rows, cols);
     return hipGetLastError();

@@ -150,6 +150,7 @@ hipError_t launch_pass(const RClass* d_cls, int step, int ntrans, int T, float2*
     const int threads = ((T + 63) / 64) * 64;
     if (threads > 1024)
         return hipErrorInvalidValue;
+    (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(k_pass, dim3(ntrans), dim3(threads), 0, st, d_cls, step, work);
     return hipGetLastError();
 }

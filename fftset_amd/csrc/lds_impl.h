@@ -154,6 +154,7 @@ hipError_t launch_one(const LdsArgs& a, const ShapeD& sh, hipStream_t st)
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
     }
     const long long grid = persist ? persistent_grid((const void*)fn, sh.threads, sh.lds_bytes, tiles) : tiles;
+    (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(sh.threads), sh.lds_bytes, st, a);
     return hipGetLastError();
 }

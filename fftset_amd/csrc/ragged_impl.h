@@ -186,6 +186,7 @@ hipError_t launch_w(const RaggedArgs& a, int lds_bytes, hipStream_t st)
     if (lds_bytes > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_ragged<THREADS>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   lds_bytes);
+    (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(k_ragged<THREADS>, dim3(a.ntiles), dim3(THREADS), lds_bytes, st, a);
     return hipGetLastError();
 }

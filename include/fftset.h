@@ -181,7 +181,9 @@ int fftset_batch_add(struct fftset_batch *b, const struct fftset_fft *fft, int o
  * FFTSET_BATCH_NO_GRAPH) captures the launches into a hipGraph. */
 int fftset_batch_commit(struct fftset_batch *b, unsigned flags);
 
-/* Runs the whole batch on a hipStream_t (stream-ordered, asynchronous). */
+/* Runs the whole batch on a hipStream_t (stream-ordered, asynchronous).
+ * Launches of one batch share its four-step work buffers: run them one at
+ * a time (on one stream, or synchronised). */
 int fftset_batch_launch(struct fftset_batch *b, void *stream);
 
 /* Runs the same work as FFTSET_BASELINE_* separate launches (for

@@ -234,3 +234,55 @@ def test_concurrent_streams_without_work_buffers(fs):
     for (y, _, _, _), x, h in zip(outs, xs, hs):
         ref = fp64.fo_conv64(x[[0, n - 1]], h[[0, n - 1]])
         assert np.max(fp64.rel_rms(y.cpu().numpy()[[0, n - 1]], ref)) <= fp64_tol(L)
+
+
+@pytest.mark.parametrize("L", [960, 61440])
+def test_drop_in_calls_from_concurrent_threads(fs, L):
+    # execution is thread-safe when out / work differ (the reference's
+    # contract): eight host threads on one plan, each with its own buffers
+    import threading
+    import torch
+    from tests.gpu_util import dev, empty, host
+    f = fs.create_fft(FO, L)
+    rng = np.random.default_rng(L)
+    xs = [rng.standard_normal(2 * L).astype(np.float32) for _ in range(8)]
+    hs = [rng.standard_normal(2 * L).astype(np.float32) for _ in range(8)]
+    expect = [(gpu_forward(f, x), gpu_conv(f, x, h)) for x, h in zip(xs, hs)]
+    bufs = [(dev(x), dev(h), empty(2 * L), empty(2 * L), empty(2 * L), empty(2 * L)) for x, h in zip(xs, hs)]
+    torch.cuda.synchronize()
+    errors = []
+
+    def worker(i):
+        xi, hi, out, k, y, work = bufs[i]
+        try:
+            for _ in range(10):
+                f.forward(out, xi, work)
+                f.conv_get_kernel(k, hi)
+                f.conv(y, xi, k, work)
+        except Exception as e:  # pragma: no cover
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors
+    for (xi, hi, out, k, y, work), (ef, ec) in zip(bufs, expect):
+        assert np.array_equal(host(out), ef)
+        assert np.array_equal(host(y), ec)
+
+
+def test_allocation_failure_is_reported(fs):
+    # a work-less four-step batch far beyond HBM: the scratch allocation fails
+    # and is reported before any kernel is launched; the set stays usable
+    import torch
+    L = 61440
+    f = fs.create_fft(CX, L)
+    x = torch.zeros(2 * L, device="cuda")
+    with pytest.raises(fftset_amd.FFTSetError) as e:
+        f.execute(fftset_amd.OP_FORWARD, 1 << 26, x, x, out_stride=2 * L, in_stride=2 * L)
+    assert e.value.code == fftset_amd.E_NOMEM
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(2 * L).astype(np.float32)
+    assert fp64.rel_rms(gpu_forward(f, v), fp64.cx_forward64(v)) <= fp64_tol(L)
