@@ -17,6 +17,8 @@ Workloads (complex_bins = L):
   c2       COMPLEX L=3840 x 65536, forward + inverse
   c3       FREQOFFSETREAL conv L=4096 x 16384 channels
   c5       FREQOFFSETREAL conv L=2048 x 4096 + both adjoints
+  large    the ragged batch's four-step share alone: L=61440, 1250 COMPLEX
+           forward + 1250 FREQOFFSETREAL conv (two launches each)
   block    overlap-save FIR filter, 1024 taps, 64 channels x 4 Mi samples
            (fftset_recommend_conv_length(1024, 0) = 8192 bins; algorithmic
            bytes: 8 per sample, signal in and output out)
@@ -48,7 +50,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="ragged", choices=["ragged", "c2", "c3", "c5", "block"])
+    ap.add_argument("--workload", default="ragged", choices=["ragged", "c2", "c3", "c5", "large", "block"])
     ap.add_argument("--transforms", type=int, default=20000, help="ragged: batch size")
     ap.add_argument("--threads", type=int, default=256, choices=[256, 512], help="ragged: lanes per workgroup")
     ap.add_argument("--no-graph", action="store_true", help="ragged: launch phases directly")
@@ -188,6 +190,44 @@ def workload_c5(torch, fs, rng_np, args):
         return float(e)
 
     return kernels, bytes_per, cfg, check, None
+
+
+def workload_large(torch, fs, rng_np, args):
+    import numpy as np
+    import fftset_amd
+    L, B = 61440, 1250
+    fc = fs.create_fft(fftset_amd.COMPLEX, L)
+    fo = fs.create_fft(fftset_amd.FREQ_OFFSET_REAL, L)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn(B, 2 * L, device="cuda", generator=g)
+    xr = torch.randn(B, 2 * L, device="cuda", generator=g)
+    h = torch.randn(B, 2 * L, device="cuda", generator=g)
+    K = torch.empty_like(h)
+    fo.execute(fftset_amd.OP_GET_KERNEL, B, K, h)
+    y, yr = torch.empty_like(x), torch.empty_like(x)
+    wc, wr = torch.empty_like(x), torch.empty_like(x)
+    kernels = {
+        "forward_cx": lambda: fc.execute(fftset_amd.OP_FORWARD, B, y, x, work=wc),
+        "conv_fo": lambda: fo.execute(fftset_amd.OP_CONV, B, yr, xr, kern=K, work=wr),
+    }
+    bytes_per = {"forward_cx": 16 * L * B, "conv_fo": 24 * L * B}
+    cfg = {"workload": "four-step transforms: COMPLEX forward + FREQOFFSETREAL conv at 61440 bins",
+           "complex_bins": L, "batch_each": B, "radices_cx": fc.radices(), "radices_fo": fo.radices(),
+           "transforms_per_step": 2 * B, "check_against": "fp64",
+           "nominal_flop_per_step": B * (nominal_flop(L, "fwd") + nominal_flop(L, "conv"))}
+
+    def check():
+        from oracle import fp64
+        idx = [0, B - 1]
+        e1 = fp64.rel_rms(y[idx].cpu().numpy(), fp64.cx_forward64(x[idx].cpu().numpy())).max()
+        e2 = fp64.rel_rms(yr[idx].cpu().numpy(),
+                          fp64.fo_conv64(xr[idx].cpu().numpy(), h[idx].cpu().numpy())).max()
+        return float(max(e1, e2))
+
+    def cpu_sample(seconds):
+        return cpu_baseline_uniform(x, L, "cx", ["forward"], seconds)
+
+    return kernels, bytes_per, cfg, check, cpu_sample
 
 
 def workload_block(torch, fs, rng_np, args):
@@ -483,7 +523,7 @@ def main():
     fs = fftset_amd.FFTSet(tune_file=tune)
     rng = np.random.default_rng(0)
     build = {"ragged": workload_ragged, "c2": workload_c2, "c3": workload_c3, "c5": workload_c5,
-             "block": workload_block}[args.workload]
+             "large": workload_large, "block": workload_block}[args.workload]
     built = build(torch, fs, rng, args)
     kernels, bytes_per, cfg, check, cpu_sample = built[:5]
     baselines = built[5] if len(built) > 5 else None
