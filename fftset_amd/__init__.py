@@ -234,6 +234,7 @@ BATCH_NO_GRAPH = 1
 BATCH_THREADS_512 = 2
 BATCH_SCHED_SERIAL = 4
 BATCH_SCHED_PHASED = 8
+BATCH_PERSISTENT = 16
 BASELINE_PER_GROUP = 1
 BASELINE_PER_TRANSFORM = 2
 BASELINE_PER_PASS = 3

@@ -49,6 +49,7 @@ struct TileRange {
     int first, count;
     int threads = 256;
     int lds_bytes = 0;
+    int counter = -1;  // persistent launches: index of its tile counter
 };
 
 }  // namespace fsa
@@ -72,6 +73,8 @@ struct fftset_batch {
     std::vector<TileRange> lds_r;    // LDS-resident tiles, one range per workgroup size
     std::vector<TileRange> fs_r[3];  // four-step steps (columns, rows, closing columns)
     int sched = 0;                   // FFTSET_BATCH_SCHED_*
+    bool persistent = false;         // FFTSET_BATCH_PERSISTENT
+    int* d_counters = nullptr;       // one tile counter per fused range
     std::vector<TileRange> group_ranges;
     std::vector<TileRange> job_ranges;
     float2* pass_work = nullptr;     // per-pass baseline work tiles (allocated on first use)
@@ -314,6 +317,12 @@ static int commit(fftset_batch* b, hipStream_t st)
         per_job.insert(per_job.end(), tl.begin(), tl.end());
     }
 
+    // tile counters of the fused ranges (persistent launches)
+    int ncounters = 0;
+    for (TileRange& r : b->lds_r) r.counter = ncounters++;
+    for (int q = 0; q < 3; ++q)
+        for (TileRange& r : b->fs_r[q]) r.counter = ncounters++;
+
     // device memory: classes, three tile tables, per-class pointer arrays.
     // Lay out every region first (16-byte aligned), then allocate.
     size_t off = 0;
@@ -326,6 +335,7 @@ static int commit(fftset_batch* b, hipStream_t st)
     const size_t o_f = place(sizeof(RTile) * fused.size());
     const size_t o_g = place(sizeof(RTile) * group.size());
     const size_t o_j = place(sizeof(RTile) * per_job.size());
+    const size_t o_c = place(sizeof(int) * (size_t)std::max(1, ncounters));
     std::vector<size_t> o_ptr;  // four arrays per class: in, out, kern, work
     for (auto& k : b->classes)
         for (int q = 0; q < 4; ++q) o_ptr.push_back(place(sizeof(void*) * k.jobs.size()));
@@ -376,6 +386,7 @@ static int commit(fftset_batch* b, hipStream_t st)
     b->d_tiles = (RTile*)(dbase + o_f);
     b->d_group_tiles = (RTile*)(dbase + o_g);
     b->d_job_tiles = (RTile*)(dbase + o_j);
+    b->d_counters = (int*)(dbase + o_c);
     return FFTSET_OK;
 }
 
@@ -383,7 +394,8 @@ static int launch_range(const fftset_batch* b, const RTile* tiles, TileRange r, 
 {
     if (r.count <= 0)
         return FFTSET_OK;
-    RaggedArgs a{b->d_classes, tiles + r.first, r.count};
+    RaggedArgs a{b->d_classes, tiles + r.first, r.count,
+                 b->persistent && r.counter >= 0 && tiles == b->d_tiles ? b->d_counters + r.counter : nullptr};
     return launch_ragged(a, r.threads, r.lds_bytes, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
 }
 
@@ -511,6 +523,7 @@ int fftset_batch_commit(struct fftset_batch* b, unsigned flags)
                : (flags & FFTSET_BATCH_SCHED_PHASED) ? FFTSET_BATCH_SCHED_PHASED
                                                      : 0;
     b->use_graph = !(flags & FFTSET_BATCH_NO_GRAPH);
+    b->persistent = (flags & FFTSET_BATCH_PERSISTENT) != 0;
     hipStream_t st = thread_stream(b->ctx->device);
     rc = commit(b, st);
     if (rc == FFTSET_OK && b->use_graph) {

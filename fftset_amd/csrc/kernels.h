@@ -78,6 +78,7 @@ struct RaggedArgs {
     const RClass* classes;
     const RTile* tiles;
     int ntiles;
+    int* next;  // persistent launch: tile counter (zeroed by the launcher); null: one tile per workgroup
 };
 
 // Launches tiles [0, ntiles) of `a`; every class must fit `threads` lanes.

@@ -158,11 +158,9 @@ __device__ __forceinline__ void ragged_dispatch_kind(const RClass& k, const RTil
     }
 }
 
-template <int THREADS>
-__global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
+__device__ __forceinline__ void ragged_tile(const RaggedArgs& a, int ti, float2* smem)
 {
-    extern __shared__ float2 smem[];
-    const RTile t = a.tiles[blockIdx.x];
+    const RTile t = a.tiles[ti];
     const RClass& k = a.classes[t.cls];
     const int sel = k.mod * OP_COUNT + k.op;
     switch (sel) {
@@ -180,14 +178,55 @@ __global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
     }
 }
 
+// One tile per workgroup (grid = tiles), or persistent: a resident grid
+// whose workgroups take tiles in table order from a counter, so the
+// largest-first order holds across the whole launch. Tiles of one launch
+// never depend on each other, so no workgroup ever waits on another.
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
+{
+    extern __shared__ float2 smem[];
+    __shared__ int s_tile;
+    int ti = blockIdx.x;
+    for (;;) {  // one call site of the (large, inlined) tile body
+        if (a.next) {
+            if (threadIdx.x == 0)
+                s_tile = atomicAdd(a.next, 1);
+            __syncthreads();
+            ti = s_tile;
+            if (ti >= a.ntiles)
+                return;  // the same value in every lane: the whole workgroup leaves
+        }
+        ragged_tile(a, ti, smem);
+        if (!a.next)
+            return;
+        __syncthreads();  // LDS and s_tile are reused by the next tile
+    }
+}
+
+template <int THREADS>
+__global__ void k_ragged_reset(int* next)
+{
+    *next = 0;
+}
+
 template <int THREADS>
 hipError_t launch_w(const RaggedArgs& a, int lds_bytes, hipStream_t st)
 {
     if (lds_bytes > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_ragged<THREADS>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   lds_bytes);
+    long long grid = a.ntiles;
+    if (a.next) {
+        // a kernel node, not a memset: the counter is reset inside captured graphs too
+        (void)hipGetLastError();
+        hipLaunchKernelGGL(k_ragged_reset<THREADS>, dim3(1), dim3(1), 0, st, a.next);
+        if (hipGetLastError() != hipSuccess)
+            return hipErrorLaunchFailure;
+        grid = persistent_grid((const void*)k_ragged<THREADS>, THREADS, lds_bytes, a.ntiles);
+    }
     (void)hipGetLastError();  // report this launch, not an earlier failed call
-    hipLaunchKernelGGL(k_ragged<THREADS>, dim3(a.ntiles), dim3(THREADS), lds_bytes, st, a);
+    hipLaunchKernelGGL(k_ragged<THREADS>, dim3((unsigned)grid), dim3(THREADS), lds_bytes, st, a);
     return hipGetLastError();
 }
 

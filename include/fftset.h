@@ -161,6 +161,8 @@ struct fftset_batch;
 #define FFTSET_BATCH_THREADS_512   2u  /* at least 512-lane workgroups (default 256)                */
 #define FFTSET_BATCH_SCHED_SERIAL  4u  /* run the tile groups one after another                      */
 #define FFTSET_BATCH_SCHED_PHASED  8u  /* LDS groups beside the first four-step step only            */
+#define FFTSET_BATCH_PERSISTENT   16u  /* resident-size grids taking tiles from a counter, instead of
+                                            one workgroup per tile                                     */
 
 #define FFTSET_BASELINE_PER_GROUP      1  /* one launch per (plan, operation) group and step */
 #define FFTSET_BASELINE_PER_TRANSFORM  2  /* one launch per transform and step               */

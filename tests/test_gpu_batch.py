@@ -97,7 +97,9 @@ SPEC = [
 
 
 @pytest.mark.parametrize("flags", [0, fftset_amd.BATCH_NO_GRAPH, fftset_amd.BATCH_THREADS_512,
-                                   fftset_amd.BATCH_SCHED_SERIAL, fftset_amd.BATCH_SCHED_PHASED])
+                                   fftset_amd.BATCH_SCHED_SERIAL, fftset_amd.BATCH_SCHED_PHASED,
+                                   fftset_amd.BATCH_PERSISTENT,
+                                   fftset_amd.BATCH_PERSISTENT | fftset_amd.BATCH_NO_GRAPH])
 def test_ragged_batch_matches_oracle(fs, oracle, flags):
     rng = np.random.default_rng(11 + flags)
     jobs = _make_jobs(rng, SPEC)
@@ -170,6 +172,31 @@ def test_per_pass_baseline_matches_the_fused_launch(fs, oracle):
             if jb["op"] in (fftset_amd.OP_FORWARD, fftset_amd.OP_INVERSE, fftset_amd.OP_CONV):
                 assert fp64.rel_rms(y, _oracle_ref(oracle, jb)) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
     b.destroy()
+
+
+def test_persistent_graph_replays_match_one_tile_per_workgroup(fs):
+    # the tile counters are reset inside the graph: every replay redoes
+    # every tile, and the results equal the one-tile-per-workgroup launch
+    rng = np.random.default_rng(21)
+    jobs = _make_jobs(rng, SPEC)
+    b = _build(fs, jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    ref = [jb["dy"].cpu().numpy().copy() for jb in jobs]
+    b.destroy()
+    bp = fs.batch()
+    for jb in jobs:
+        bp.add(jb["f"], jb["op"], jb["dy"], jb["dx"], jb["dk"])
+    bp.commit(fftset_amd.BATCH_PERSISTENT)
+    for rep in range(3):
+        for jb in jobs:
+            jb["dy"].fill_(float("nan"))
+        torch.cuda.synchronize()
+        bp.launch()
+        torch.cuda.synchronize()
+        for jb, r in zip(jobs, ref):
+            assert np.array_equal(jb["dy"].cpu().numpy(), r), (rep, jb["mod"], jb["L"], jb["op"])
+    bp.destroy()
 
 
 def test_batch_rejects_bad_use(fs):
