@@ -87,8 +87,18 @@ int persistent_waves()
     }
     return w;
 }
+hipError_t launch_lds_fixed(int mod, int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st);
+
+// Chains with a compile-time specialisation (kernels_fixed.hip) use it,
+// unless FFTSET_AMD_GENERIC=1 (read per call) asks for the generic kernel.
 hipError_t launch_lds(int mod, int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st)
 {
+    const char* g = getenv("FFTSET_AMD_GENERIC");
+    if (!persistent_mode() && !(g && atoi(g) > 0)) {
+        const hipError_t e = launch_lds_fixed(mod, op, a, sh, st);
+        if (e != hipErrorNotSupported)
+            return e;
+    }
     return mod == MOD_FO ? launch_lds_fo(op, a, sh, st) : launch_lds_cx(op, a, sh, st);
 }
 

@@ -286,3 +286,39 @@ def test_allocation_failure_is_reported(fs):
     rng = np.random.default_rng(5)
     v = rng.standard_normal(2 * L).astype(np.float32)
     assert fp64.rel_rms(gpu_forward(f, v), fp64.cx_forward64(v)) <= fp64_tol(L)
+
+
+@pytest.mark.parametrize("mod,L,op", [(CX, 3840, "fwd"), (CX, 3840, "inv"), (FO, 4096, "conv"), (FO, 4096, "kern"),
+                                      (FO, 2048, "conv"), (FO, 2048, "grad"), (FO, 2048, "kern")])
+def test_specialised_kernels_match_generic(fs, monkeypatch, mod, L, op):
+    # compile-time specialised chains (kernels_fixed.hip) against the
+    # generic runtime-chain kernel on the same plan and inputs
+    import torch
+    f = fs.create_fft(mod, L)
+    n = 64
+    rng = np.random.default_rng(L + len(op))
+    x = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
+    h = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
+    k = torch.empty_like(h)
+    f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
+
+    def run():
+        y = torch.empty_like(x)
+        if op == "fwd":
+            f.execute(fftset_amd.OP_FORWARD, n, y, x)
+        elif op == "inv":
+            f.execute(fftset_amd.OP_INVERSE, n, y, x)
+        elif op == "kern":
+            f.execute(fftset_amd.OP_GET_KERNEL, n, y, x)
+        elif op == "conv":
+            f.execute(fftset_amd.OP_CONV, n, y, x, kern=k)
+        else:
+            f.execute(fftset_amd.OP_CONV_GRAD_INPUT, n, y, x, kern=k)
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    fixed = run()
+    monkeypatch.setenv("FFTSET_AMD_GENERIC", "1")
+    generic = run()
+    assert np.all(np.isfinite(fixed))
+    assert np.max(fp64.rel_rms(fixed, generic)) <= 1e-6
