@@ -50,6 +50,7 @@ struct TileRange {
     int threads = 256;
     int lds_bytes = 0;
     int counter = -1;  // persistent launches: index of its tile counter
+    bool four_step = false;  // four-step tiles (else LDS tiles)
 };
 
 }  // namespace fsa
@@ -275,8 +276,8 @@ static int commit(fftset_batch* b, hipStream_t st)
                 continue;
             const int kind = g3[q][0].kind;
             b->classes[ci].group_idx.push_back((int)b->group_ranges.size());
-            b->group_ranges.push_back(
-                {(int)group.size(), (int)g3[q].size(), tile_threads(k, kind), tile_lds_bytes(k, kind)});
+            b->group_ranges.push_back({(int)group.size(), (int)g3[q].size(), tile_threads(k, kind),
+                                       tile_lds_bytes(k, kind), -1, kind != TK_LDS});
             group.insert(group.end(), g3[q].begin(), g3[q].end());
         }
     }
@@ -285,7 +286,7 @@ static int commit(fftset_batch* b, hipStream_t st)
     std::vector<RTile> fused;
     auto add_ranges = [&](const std::vector<CostTile>& src, bool want_lds, std::vector<TileRange>& dst) {
         for (int w = 256; w <= 1024; w *= 2) {
-            TileRange r{(int)fused.size(), 0, w, 0};
+            TileRange r{(int)fused.size(), 0, w, 0, -1, !want_lds};
             for (auto& ct : src) {
                 const BatchClass& k = b->classes[ct.t.cls];
                 if (tile_threads(k, ct.t.kind) != w || (ct.t.kind == TK_LDS) != want_lds)
@@ -312,6 +313,7 @@ static int commit(fftset_batch* b, hipStream_t st)
             const BatchClass& k = b->classes[tl[0].cls];
             r.threads = tile_threads(k, tl[0].kind);
             r.lds_bytes = tile_lds_bytes(k, tl[0].kind);
+            r.four_step = tl[0].kind != TK_LDS;
         }
         b->job_ranges.push_back(r);
         per_job.insert(per_job.end(), tl.begin(), tl.end());
@@ -395,7 +397,8 @@ static int launch_range(const fftset_batch* b, const RTile* tiles, TileRange r, 
     if (r.count <= 0)
         return FFTSET_OK;
     RaggedArgs a{b->d_classes, tiles + r.first, r.count,
-                 b->persistent && r.counter >= 0 && tiles == b->d_tiles ? b->d_counters + r.counter : nullptr};
+                 b->persistent && r.counter >= 0 && tiles == b->d_tiles ? b->d_counters + r.counter : nullptr,
+                 r.four_step ? 1 : 0};
     return launch_ragged(a, r.threads, r.lds_bytes, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
 }
 

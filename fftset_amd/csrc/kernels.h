@@ -79,6 +79,7 @@ struct RaggedArgs {
     const RTile* tiles;
     int ntiles;
     int* next;  // persistent launch: tile counter (zeroed by the launcher); null: one tile per workgroup
+    int four_step;  // every tile is a four-step tile (else every tile is an LDS tile)
 };
 
 // Launches tiles [0, ntiles) of `a`; every class must fit `threads` lanes.

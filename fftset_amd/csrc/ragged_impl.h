@@ -143,37 +143,43 @@ __device__ __forceinline__ void ragged_row(const RClass& k, const RTile& t, floa
     }
 }
 
-template <int MOD, int OP>
+// LDS tiles and four-step tiles are compiled into separate kernels (each
+// launch range holds one kind), which keeps each kernel's code small.
+template <int MOD, int OP, bool FS>
 __device__ __forceinline__ void ragged_dispatch_kind(const RClass& k, const RTile& t, float2* smem)
 {
-    switch (t.kind) {
-    case TK_LDS: ragged_lds<MOD, OP>(k, t, smem); break;
-    case TK_COL: ragged_col<MOD, OP, false>(k, t, smem); break;
-    case TK_ROW: ragged_row<MOD, OP>(k, t, smem); break;
-    case TK_COL2:
-        if constexpr (OP == OP_CONV || OP == OP_GRAD_IN)
-            ragged_col<MOD, OP, true>(k, t, smem);
-        break;
-    default: break;
+    if constexpr (!FS) {
+        ragged_lds<MOD, OP>(k, t, smem);
+    } else {
+        switch (t.kind) {
+        case TK_COL: ragged_col<MOD, OP, false>(k, t, smem); break;
+        case TK_ROW: ragged_row<MOD, OP>(k, t, smem); break;
+        case TK_COL2:
+            if constexpr (OP == OP_CONV || OP == OP_GRAD_IN)
+                ragged_col<MOD, OP, true>(k, t, smem);
+            break;
+        default: break;
+        }
     }
 }
 
+template <bool FS>
 __device__ __forceinline__ void ragged_tile(const RaggedArgs& a, int ti, float2* smem)
 {
     const RTile t = a.tiles[ti];
     const RClass& k = a.classes[t.cls];
     const int sel = k.mod * OP_COUNT + k.op;
     switch (sel) {
-    case MOD_FO * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_FO, OP_FWD>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_FO, OP_INV>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_FO, OP_KERN>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_FO, OP_CONV>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_FO, OP_GRAD_IN>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_CX, OP_FWD>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_CX, OP_INV>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_CX, OP_KERN>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_CX, OP_CONV>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_CX, OP_GRAD_IN>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_FO, OP_FWD, FS>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_FO, OP_INV, FS>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_FO, OP_KERN, FS>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_FO, OP_CONV, FS>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_FO, OP_GRAD_IN, FS>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_CX, OP_FWD, FS>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_CX, OP_INV, FS>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_CX, OP_KERN, FS>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_CX, OP_CONV, FS>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_CX, OP_GRAD_IN, FS>(k, t, smem); break;
     default: break;
     }
 }
@@ -182,7 +188,7 @@ __device__ __forceinline__ void ragged_tile(const RaggedArgs& a, int ti, float2*
 // whose workgroups take tiles in table order from a counter, so the
 // largest-first order holds across the whole launch. Tiles of one launch
 // never depend on each other, so no workgroup ever waits on another.
-template <int THREADS>
+template <int THREADS, bool FS>
 __global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
 {
     extern __shared__ float2 smem[];
@@ -197,7 +203,7 @@ __global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
             if (ti >= a.ntiles)
                 return;  // the same value in every lane: the whole workgroup leaves
         }
-        ragged_tile(a, ti, smem);
+        ragged_tile<FS>(a, ti, smem);
         if (!a.next)
             return;
         __syncthreads();  // LDS and s_tile are reused by the next tile
@@ -213,9 +219,9 @@ __global__ void k_ragged_reset(int* next)
 template <int THREADS>
 hipError_t launch_w(const RaggedArgs& a, int lds_bytes, hipStream_t st)
 {
+    auto fn = a.four_step ? k_ragged<THREADS, true> : k_ragged<THREADS, false>;
     if (lds_bytes > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_ragged<THREADS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  lds_bytes);
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
     long long grid = a.ntiles;
     if (a.next) {
         // a kernel node, not a memset: the counter is reset inside captured graphs too
@@ -223,10 +229,10 @@ hipError_t launch_w(const RaggedArgs& a, int lds_bytes, hipStream_t st)
         hipLaunchKernelGGL(k_ragged_reset<THREADS>, dim3(1), dim3(1), 0, st, a.next);
         if (hipGetLastError() != hipSuccess)
             return hipErrorLaunchFailure;
-        grid = persistent_grid((const void*)k_ragged<THREADS>, THREADS, lds_bytes, a.ntiles);
+        grid = persistent_grid((const void*)fn, THREADS, lds_bytes, a.ntiles);
     }
     (void)hipGetLastError();  // report this launch, not an earlier failed call
-    hipLaunchKernelGGL(k_ragged<THREADS>, dim3((unsigned)grid), dim3(THREADS), lds_bytes, st, a);
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(THREADS), lds_bytes, st, a);
     return hipGetLastError();
 }
 
