@@ -508,7 +508,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has a single MI355X and this path has no "
-              "multi-GPU component (transforms are independent; see DESIGN.md for how it would shard).",
+              "multi-GPU component (transforms are independent: a batch would shard across GPUs by bytes, with no "
+              "exchange step).",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
