@@ -235,6 +235,7 @@ BATCH_THREADS_512 = 2
 BATCH_SCHED_SERIAL = 4
 BATCH_SCHED_PHASED = 8
 BATCH_PERSISTENT = 16
+BATCH_FOUR_STEP_TILES = 32
 BASELINE_PER_GROUP = 1
 BASELINE_PER_TRANSFORM = 2
 BASELINE_PER_PASS = 3

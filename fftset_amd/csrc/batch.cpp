@@ -5,12 +5,13 @@
 // each class either LDS-resident tiles (G transforms per workgroup) or a
 // four-step decomposition, builds the tile table sorted largest first, and
 // captures the launches into a hipGraph:
-//   phase 1: every LDS tile + the first column step of four-step transforms
-//   phase 2: the row steps of four-step transforms
-//   phase 3: the closing column step of four-step convolutions
-// Each phase is one launch of the ragged kernel over its tiles. The same
-// kernel launched once per (class, phase), or once per transform, gives the
-// per-size-launch baselines.
+//   LDS tiles: one launch of the ragged kernel per workgroup size;
+//   four-step classes: columns -> rows -> closing columns (convolutions),
+//     each step one launch of the four-step kernels per class, reading the
+//     class's per-transform pointer arrays (FFTSET_BATCH_FOUR_STEP_TILES:
+//     one launch of the ragged kernel per step over all four-step classes).
+// The same kernels launched once per (class, step), or once per transform,
+// give the per-size-launch baselines.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,6 +52,7 @@ struct TileRange {
     int lds_bytes = 0;
     int counter = -1;  // persistent launches: index of its tile counter
     bool four_step = false;  // four-step tiles (else LDS tiles)
+    int cls = -1, job = -1, step = -1;  // per-transform ranges: class, transform, four-step step
 };
 
 }  // namespace fsa
@@ -75,6 +77,7 @@ struct fftset_batch {
     std::vector<TileRange> fs_r[3];  // four-step steps (columns, rows, closing columns)
     int sched = 0;                   // FFTSET_BATCH_SCHED_*
     bool persistent = false;         // FFTSET_BATCH_PERSISTENT
+    bool fs_tiles = false;           // four-step steps as ragged tiles (else the four-step kernels)
     int* d_counters = nullptr;       // one tile counter per fused range
     std::vector<TileRange> group_ranges;
     std::vector<TileRange> job_ranges;
@@ -314,6 +317,9 @@ static int commit(fftset_batch* b, hipStream_t st)
             r.threads = tile_threads(k, tl[0].kind);
             r.lds_bytes = tile_lds_bytes(k, tl[0].kind);
             r.four_step = tl[0].kind != TK_LDS;
+            r.cls = tl[0].cls;
+            r.job = tl[0].job;
+            r.step = (int)(v % 3);
         }
         b->job_ranges.push_back(r);
         per_job.insert(per_job.end(), tl.begin(), tl.end());
@@ -376,6 +382,7 @@ static int commit(fftset_batch* b, hipStream_t st)
         r.kern = (const float* const*)(dbase + o_k);
         r.work = (float* const*)(dbase + o_w);
         cls.push_back(r);
+        b->classes[ci].rc = r;  // the four-step launches take the device pointer arrays
     }
     std::memcpy(host.data() + o_cls, cls.data(), sizeof(RClass) * cls.size());
     std::memcpy(host.data() + o_f, fused.data(), sizeof(RTile) * fused.size());
@@ -412,6 +419,77 @@ static int launch_list(const fftset_batch* b, const std::vector<TileRange>& rs, 
     return FFTSET_OK;
 }
 
+// Step q (0 columns, 1 rows, 2 closing columns) of a four-step class through
+// the drop-in four-step kernels, which read the class's per-transform pointer
+// arrays: the same kernels, tile shapes and data order as run_four_step.
+// j0 / n: a sub-range of the class's transforms (n < 0: all of them).
+static int launch_fs_class(const BatchClass& k, int q, hipStream_t st, long long j0 = 0, long long n = -1)
+{
+    RClass r = k.rc;
+    r.in += j0;
+    r.out += j0;
+    r.kern += j0;
+    r.work += j0;
+    const FourStep& fs = k.fs;
+    const bool fo = r.mod == MOD_FO;
+    const bool conv = r.op == OP_CONV || r.op == OP_GRAD_IN;
+    if (q == 2 && !conv)
+        return FFTSET_OK;
+    FsArgs a{};
+    a.tmod = r.tmod;
+    a.tfs = r.tfs;
+    a.L = r.L;
+    a.A = r.A;
+    a.B = r.B;
+    a.ntrans = n < 0 ? (long long)k.jobs.size() - j0 : n;
+    a.conj_kern = r.op == OP_GRAD_IN;
+    const ShapeD& sh = q == 1 ? fs.rows : fs.cols;
+    a.c = q == 1 ? fs.rowc : fs.colc;
+    a.C = sh.G;
+    a.T = sh.T;
+    a.groups = q == 1 ? (r.A + a.C - 1) / a.C : (r.B + a.C - 1) / a.C;
+    hipError_t e;
+    if (q == 0) {
+        a.in_p = r.in;
+        a.out_p = r.op == OP_KERN ? r.out : r.work;
+        const int load = r.op == OP_INV ? (fo ? CL_FOBINS : CL_CXCONJ) : (fo ? CL_FOMOD : CL_CX);
+        e = launch_cols(load, CS_WORKTW, a, sh, st);
+    } else if (q == 1) {
+        a.in_p = r.op == OP_KERN ? (const float* const*)r.out : (const float* const*)r.work;
+        if (conv) {
+            a.out_p = r.work;
+            a.kern_p = r.kern;
+            e = launch_rows(RK_CONV, RS_NONE, a, sh, st);
+        } else if (r.op == OP_KERN) {
+            a.out_p = r.out;
+            e = launch_rows(RK_KERN, RS_NONE, a, sh, st);
+        } else {
+            a.out_p = r.out;
+            const int store = r.op == OP_FWD ? (fo ? RS_FOBINS : RS_CX) : (fo ? RS_FODEMOD : RS_CXCONJ);
+            e = launch_rows(RK_STOCK, store, a, sh, st);
+        }
+    } else {
+        a.in_p = (const float* const*)r.work;
+        a.out_p = r.out;
+        e = launch_cols(CL_CX, fo ? CS_FODEMOD : CS_CXCONJ, a, sh, st);
+    }
+    return e == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
+}
+
+// Four-step step q of every four-step class, largest class first.
+static int launch_fs_step(const fftset_batch* b, int q, hipStream_t st)
+{
+    if (b->fs_tiles)
+        return launch_list(b, b->fs_r[q], st);
+    for (const BatchClass& k : b->classes) {
+        if (!k.four_step)
+            continue;
+        if (int rc = launch_fs_class(k, q, st))
+            return rc;
+    }
+    return FFTSET_OK;
+}
+
 // Launch schedules (the graph records the same structure):
 //  BRANCHES (default): the four-step chain (columns -> rows -> closing
 //    columns) on the main stream, each workgroup size of LDS tiles on a side
@@ -435,7 +513,7 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
             if (int rc = launch_range(b, b->d_tiles, b->lds_r[i], st))
                 return rc;
         for (int q = 0; q < 3; ++q)
-            if (int rc = launch_list(b, b->fs_r[q], st))
+            if (int rc = launch_fs_step(b, q, st))
                 return rc;
         return FFTSET_OK;
     }
@@ -450,18 +528,18 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
             return rc;
     }
     if (sched == FFTSET_BATCH_SCHED_PHASED) {
-        if (int rc = launch_list(b, b->fs_r[0], st))
+        if (int rc = launch_fs_step(b, 0, st))
             return rc;
         for (int i = 0; i < nside; ++i)
             if (!join_from(i))
                 return FFTSET_E_DEVICE;
         for (int q = 1; q < 3; ++q)
-            if (int rc = launch_list(b, b->fs_r[q], st))
+            if (int rc = launch_fs_step(b, q, st))
                 return rc;
         return FFTSET_OK;
     }
     for (int q = 0; q < 3; ++q)
-        if (int rc = launch_list(b, b->fs_r[q], st))
+        if (int rc = launch_fs_step(b, q, st))
             return rc;
     for (int i = 0; i < nside; ++i)
         if (!join_from(i))
@@ -527,6 +605,7 @@ int fftset_batch_commit(struct fftset_batch* b, unsigned flags)
                                                      : 0;
     b->use_graph = !(flags & FFTSET_BATCH_NO_GRAPH);
     b->persistent = (flags & FFTSET_BATCH_PERSISTENT) != 0;
+    b->fs_tiles = (flags & FFTSET_BATCH_FOUR_STEP_TILES) != 0;
     hipStream_t st = thread_stream(b->ctx->device);
     rc = commit(b, st);
     if (rc == FFTSET_OK && b->use_graph) {
@@ -584,9 +663,17 @@ int fftset_batch_launch_baseline(struct fftset_batch* b, int mode, void* stream)
     }
     hipStream_t st = (hipStream_t)stream;
     if (mode == FFTSET_BASELINE_PER_GROUP) {
-        for (const TileRange& r : b->group_ranges)
-            if ((rc = launch_range(b, b->d_group_tiles, r, st)) != FFTSET_OK)
-                break;
+        // one launch per (class, step), class by class
+        for (size_t ci = 0; ci < b->classes.size() && rc == FFTSET_OK; ++ci) {
+            const BatchClass& k = b->classes[ci];
+            if (k.four_step && !b->fs_tiles) {
+                for (int q = 0; q < 3 && rc == FFTSET_OK; ++q) rc = launch_fs_class(k, q, st);
+                continue;
+            }
+            for (int g : k.group_idx)
+                if ((rc = launch_range(b, b->d_group_tiles, b->group_ranges[g], st)) != FFTSET_OK)
+                    break;
+        }
     } else if (mode == FFTSET_BASELINE_PER_PASS) {
         if (!b->pass_work) {
             long long n = 0;
@@ -604,6 +691,10 @@ int fftset_batch_launch_baseline(struct fftset_batch* b, int mode, void* stream)
         for (size_t ci = 0; ci < b->classes.size() && rc == FFTSET_OK; ++ci) {
             const BatchClass& k = b->classes[ci];
             if (k.four_step) {
+                if (!b->fs_tiles) {
+                    for (int q = 0; q < 3 && rc == FFTSET_OK; ++q) rc = launch_fs_class(k, q, st);
+                    continue;
+                }
                 for (int g : k.group_idx)
                     if ((rc = launch_range(b, b->d_group_tiles, b->group_ranges[g], st)) != FFTSET_OK)
                         break;
@@ -618,9 +709,14 @@ int fftset_batch_launch_baseline(struct fftset_batch* b, int mode, void* stream)
         }
     } else if (mode == FFTSET_BASELINE_PER_TRANSFORM) {
         // one launch per transform and step (job_ranges holds 3 per transform)
-        for (const TileRange& r : b->job_ranges)
-            if ((rc = launch_range(b, b->d_job_tiles, r, st)) != FFTSET_OK)
+        for (const TileRange& r : b->job_ranges) {
+            if (r.four_step && !b->fs_tiles)
+                rc = launch_fs_class(b->classes[r.cls], r.step, st, r.job, 1);
+            else
+                rc = launch_range(b, b->d_job_tiles, r, st);
+            if (rc != FFTSET_OK)
                 break;
+        }
     } else {
         rc = FFTSET_E_INVALID;
     }

@@ -41,6 +41,11 @@ struct FsArgs {
     float* out;
     const float* kern;
     long long in_st, out_st, kern_st;
+    // Per-transform pointers (ragged batches): when set, transform t reads
+    // in_p[t] / kern_p[t] and writes out_p[t], and the strides are unused.
+    const float* const* in_p;
+    float* const* out_p;
+    const float* const* kern_p;
 };
 
 hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);

@@ -15,12 +15,29 @@
 
 namespace fsa {
 
+// Transform t's buffers: base + t * stride, or the per-transform pointer
+// arrays of a ragged batch.
+__device__ __forceinline__ const float* fs_in(const FsArgs& a, long long t)
+{
+    return a.in_p ? a.in_p[t] : a.in + t * a.in_st;
+}
+__device__ __forceinline__ float* fs_out(const FsArgs& a, long long t)
+{
+    return a.out_p ? a.out_p[t] : a.out + t * a.out_st;
+}
+__device__ __forceinline__ const float* fs_kern(const FsArgs& a, long long t)
+{
+    return a.kern_p ? a.kern_p[t] : a.kern + t * a.kern_st;
+}
+// one kernel spectrum shared by every transform (kernel stride 0)
+__device__ __forceinline__ bool fs_kern_shared(const FsArgs& a) { return !a.kern_p && a.kern_st == 0; }
+
 // Column-step functors of one (transform, column) pair.
 template <int LK, int SK>
 struct ColIo {
     static __device__ __forceinline__ auto ld(const FsArgs& a, long long t, int n2)
     {
-        const float* in = a.in + t * a.in_st;
+        const float* in = fs_in(a, t);
         if constexpr (LK == CL_FOMOD)
             return LdCol<LdFoMod>{{in, a.L, a.tmod}, a.B, n2};
         else if constexpr (LK == CL_FOBINS)
@@ -32,7 +49,7 @@ struct ColIo {
     }
     static __device__ __forceinline__ auto st(const FsArgs& a, long long t, int n2)
     {
-        float* out = a.out + t * a.out_st;
+        float* out = fs_out(a, t);
         if constexpr (SK == CS_WORKTW)
             return StWorkTw{(float2*)out, a.B, n2, a.tfs};
         else if constexpr (SK == CS_FODEMOD)
@@ -128,15 +145,15 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
         const int lt = ok ? lt0 : kIdle;
         const int k1c = ok ? k1 : 0;
         const long long t0 = t < a.ntrans ? t : 0;
-        const LdCx<false> ld{(const float2*)(a.in + t0 * a.in_st) + (long long)k1c * a.B};
-        float2* row_out = (float2*)(a.out + t0 * a.out_st) + (long long)k1c * a.B;
+        const LdCx<false> ld{(const float2*)fs_in(a, t0) + (long long)k1c * a.B};
+        float2* row_out = (float2*)fs_out(a, t0) + (long long)k1c * a.B;
         if constexpr (KIND == RK_STOCK) {
             run_stockham_to_tile<E>(a.c, tl, lt, T, ld);
             __syncthreads();
             const int nrows = min(C, a.A - k1_0);
             if (t >= a.ntrans || nrows <= 0)
                 return;
-            float* out = a.out + t * a.out_st;
+            float* out = fs_out(a, t);
             const int L = a.L, A = a.A;
             for (int idx = threadIdx.x; idx < nrows * a.B; idx += blockDim.x) {
                 const int gg = idx % nrows, k2 = idx / nrows;
@@ -152,7 +169,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
                     StCx<true>{(float2*)out}(f, v);
             }
         } else if constexpr (KIND == RK_CONV) {
-            const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st) + (long long)k1c * a.B, a.conj_kern != 0, a.kern_st == 0};
+            const BotConv bot{(const float2*)fs_kern(a, t0) + (long long)k1c * a.B, a.conj_kern != 0, fs_kern_shared(a)};
             run_conv<E>(a.c, tl, lt, T, ld, StRowTw{row_out, a.tfs + (long long)k1c * a.B}, bot);
         } else {
             run_kern<E>(a.c, tl, lt, T, ld, row_out);
@@ -175,7 +192,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
         return ok ? lt0 : kIdle;
     };
     auto row_ld = [&](long long t, int k1) {
-        return LdCx<false>{(const float2*)(a.in + t * a.in_st) + (long long)k1 * a.B};
+        return LdCx<false>{(const float2*)fs_in(a, t) + (long long)k1 * a.B};
     };
     long long tile = blockIdx.x;
     if (tile >= ntiles)
@@ -192,7 +209,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
         const int lt = lane(tile, &t, &k1);
         const long long next = tile + gridDim.x;
         const int ltn = next < ntiles ? lane(next, &tn, &k1n) : kIdle;
-        float2* row_out = (float2*)(a.out + t * a.out_st) + (long long)k1 * a.B;
+        float2* row_out = (float2*)fs_out(a, t) + (long long)k1 * a.B;
 
         if constexpr (KIND == RK_STOCK) {
             if (single)
@@ -209,7 +226,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
             const long long tt = tile / a.groups;
             const int k1_0 = (int)(tile - tt * a.groups) * C;
             const int nrows = min(C, a.A - k1_0);
-            float* out = a.out + tt * a.out_st;
+            float* out = fs_out(a, tt);
             const int L = a.L, A = a.A;
             for (int idx = threadIdx.x; idx < nrows * a.B; idx += blockDim.x) {
                 const int gg = idx % nrows, k2 = idx / nrows;
@@ -225,7 +242,7 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
                     StCx<true>{(float2*)out}(f, v);
             }
         } else if constexpr (KIND == RK_CONV) {
-            const BotConv bot{(const float2*)(a.kern + t * a.kern_st) + (long long)k1 * a.B, a.conj_kern != 0, a.kern_st == 0};
+            const BotConv bot{(const float2*)fs_kern(a, t) + (long long)k1 * a.B, a.conj_kern != 0, fs_kern_shared(a)};
             const StRowTw st{row_out, a.tfs + (long long)k1 * a.B};
             if (single) {
                 pf_single_conv<E>(a.c, pf, lt, bot, st);

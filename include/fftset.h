@@ -163,6 +163,9 @@ struct fftset_batch;
 #define FFTSET_BATCH_SCHED_PHASED  8u  /* LDS groups beside the first four-step step only            */
 #define FFTSET_BATCH_PERSISTENT   16u  /* resident-size grids taking tiles from a counter, instead of
                                             one workgroup per tile                                     */
+#define FFTSET_BATCH_FOUR_STEP_TILES 32u /* four-step steps as ragged tiles of one launch per step and
+                                            workgroup size, instead of the four-step kernels once per
+                                            (class, step) reading per-transform pointer arrays        */
 
 #define FFTSET_BASELINE_PER_GROUP      1  /* one launch per (plan, operation) group and step */
 #define FFTSET_BASELINE_PER_TRANSFORM  2  /* one launch per transform and step               */

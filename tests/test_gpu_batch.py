@@ -93,13 +93,16 @@ SPEC = [
     (FO, 61440, fftset_amd.OP_INVERSE, 1), (CX, 61440, fftset_amd.OP_CONV, 1),
     (FO, 768, fftset_amd.OP_CONV_GRAD_INPUT, 3), (CX, 360, fftset_amd.OP_CONV_GRAD_INPUT, 3),
     (FO, 61440, fftset_amd.OP_CONV_GRAD_INPUT, 1), (CX, 120, fftset_amd.OP_GET_KERNEL, 2),
+    (FO, 61440, fftset_amd.OP_GET_KERNEL, 1),
 ]
 
 
 @pytest.mark.parametrize("flags", [0, fftset_amd.BATCH_NO_GRAPH, fftset_amd.BATCH_THREADS_512,
                                    fftset_amd.BATCH_SCHED_SERIAL, fftset_amd.BATCH_SCHED_PHASED,
                                    fftset_amd.BATCH_PERSISTENT,
-                                   fftset_amd.BATCH_PERSISTENT | fftset_amd.BATCH_NO_GRAPH])
+                                   fftset_amd.BATCH_PERSISTENT | fftset_amd.BATCH_NO_GRAPH,
+                                   fftset_amd.BATCH_FOUR_STEP_TILES,
+                                   fftset_amd.BATCH_FOUR_STEP_TILES | fftset_amd.BATCH_PERSISTENT])
 def test_ragged_batch_matches_oracle(fs, oracle, flags):
     rng = np.random.default_rng(11 + flags)
     jobs = _make_jobs(rng, SPEC)
@@ -123,13 +126,14 @@ def test_ragged_batch_matches_oracle(fs, oracle, flags):
     b.destroy()
 
 
-def test_graph_replays_and_baselines_reproduce_the_launch(fs):
+@pytest.mark.parametrize("flags", [0, fftset_amd.BATCH_FOUR_STEP_TILES])
+def test_graph_replays_and_baselines_reproduce_the_launch(fs, flags):
     rng = np.random.default_rng(5)
     spec = [(CX, 48, fftset_amd.OP_FORWARD, 16), (FO, 768, fftset_amd.OP_CONV, 6),
-            (FO, 4608, fftset_amd.OP_CONV, 2), (CX, 61440, fftset_amd.OP_FORWARD, 1),
-            (FO, 61440, fftset_amd.OP_CONV, 1)]
+            (FO, 4608, fftset_amd.OP_CONV, 2), (CX, 61440, fftset_amd.OP_FORWARD, 3),
+            (FO, 61440, fftset_amd.OP_CONV, 2)]
     jobs = _make_jobs(rng, spec)
-    b = _build(fs, jobs)
+    b = _build(fs, jobs, flags)
     b.launch()
     torch.cuda.synchronize()
     first = [jb["dy"].cpu().numpy().copy() for jb in jobs]
@@ -171,6 +175,28 @@ def test_per_pass_baseline_matches_the_fused_launch(fs, oracle):
             assert fp64.rel_rms(y, fused) <= 1e-6, (jb["mod"], jb["L"], jb["op"])
             if jb["op"] in (fftset_amd.OP_FORWARD, fftset_amd.OP_INVERSE, fftset_amd.OP_CONV):
                 assert fp64.rel_rms(y, _oracle_ref(oracle, jb)) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
+    b.destroy()
+
+
+def test_batch_four_step_classes_equal_drop_in_calls(fs):
+    # four-step classes run the drop-in four-step kernels through the
+    # class's pointer arrays: outputs equal single-transform drop-in calls bit
+    # for bit, whatever the order and addresses of the transforms
+    rng = np.random.default_rng(17)
+    spec = [(CX, 61440, fftset_amd.OP_FORWARD, 2), (FO, 61440, fftset_amd.OP_CONV, 2),
+            (FO, 61440, fftset_amd.OP_INVERSE, 1), (CX, 61440, fftset_amd.OP_CONV_GRAD_INPUT, 1),
+            (FO, 61440, fftset_amd.OP_GET_KERNEL, 1), (CX, 768, fftset_amd.OP_FORWARD, 3)]
+    jobs = _make_jobs(rng, spec)
+    b = _build(fs, jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    for jb in jobs:
+        if jb["L"] != 61440:
+            continue
+        ref = torch.full_like(jb["dx"], float("nan"))
+        jb["f"].execute(jb["op"], 1, ref, jb["dx"], kern=jb["dk"])
+        torch.cuda.synchronize()
+        assert np.array_equal(jb["dy"].cpu().numpy(), ref.cpu().numpy()), (jb["mod"], jb["op"])
     b.destroy()
 
 
