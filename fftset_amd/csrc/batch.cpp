@@ -491,9 +491,12 @@ static int launch_fs_step(const fftset_batch* b, int q, hipStream_t st)
 }
 
 // Launch schedules (the graph records the same structure):
-//  BRANCHES (default): the four-step chain (columns -> rows -> closing
-//    columns) on the main stream, each workgroup size of LDS tiles on a side
-//    branch forked at the start and joined at the end;
+//  BRANCHES (default): each workgroup size of LDS tiles on a side branch
+//    forked at the start and joined at the end; each four-step class's chain
+//    (columns -> rows -> closing columns) on a branch of its own, the largest
+//    on the main stream, the others after the LDS groups on the side branches
+//    (with FFTSET_BATCH_FOUR_STEP_TILES: one chain of tile launches on the
+//    main stream);
 //  PHASED: the LDS groups and the four-step column step run concurrently,
 //    then the rows, then the closing columns;
 //  SERIAL: every group one after the other, largest workgroups first.
@@ -538,10 +541,40 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
                 return rc;
         return FFTSET_OK;
     }
-    for (int q = 0; q < 3; ++q)
-        if (int rc = launch_fs_step(b, q, st))
-            return rc;
-    for (int i = 0; i < nside; ++i)
+    int used = nside;  // side streams forked so far
+    if (b->fs_tiles) {
+        for (int q = 0; q < 3; ++q)
+            if (int rc = launch_fs_step(b, q, st))
+                return rc;
+    } else {
+        // one chain per four-step class: the largest on the main stream, the
+        // others on the side branches, after the LDS group there
+        std::vector<const BatchClass*> fsc;
+        for (const BatchClass& k : b->classes)
+            if (k.four_step)
+                fsc.push_back(&k);
+        std::stable_sort(fsc.begin(), fsc.end(), [](const BatchClass* x, const BatchClass* y) {
+            return op_bytes(x->op, x->rc.L) * (long long)x->jobs.size() >
+                   op_bytes(y->op, y->rc.L) * (long long)y->jobs.size();
+        });
+        for (size_t c = 0; c < fsc.size(); ++c) {
+            hipStream_t s = st;
+            if (c > 0) {
+                const int i = (int)((c - 1) % 3);
+                if (i >= used) {
+                    for (int u = used; u <= i; ++u)
+                        if (!fork_to(u))
+                            return FFTSET_E_DEVICE;
+                    used = i + 1;
+                }
+                s = b->side[i];
+            }
+            for (int q = 0; q < 3; ++q)
+                if (int rc = launch_fs_class(*fsc[c], q, s))
+                    return rc;
+        }
+    }
+    for (int i = 0; i < used; ++i)
         if (!join_from(i))
             return FFTSET_E_DEVICE;
     return FFTSET_OK;
