@@ -11,6 +11,7 @@
 // signal and the output cross HBM once; only the k - 1 overlap samples per
 // block are read twice.
 #include <algorithm>
+#include <cstdlib>
 
 #include "drivers.h"
 #include "kernels.h"
@@ -79,6 +80,62 @@ __global__ void __launch_bounds__(1024) k_block(const BlockArgs a)
     run_conv<E>(a.c, tl, lt, T, ld, st, bot);
 }
 
+// The same frame convolution with a four-pass chain [R0, R1, R2, R3] and the
+// lanes per transform as template constants (strides, bounds and LDS offsets
+// fold at compile time), for the tuned chain of the block size.
+template <int R0, int R1, int R2, int R3, int T>
+__global__ void __launch_bounds__(1024) k_block_fix(const BlockArgs a)
+{
+    constexpr int E = 16;
+    constexpr int n = R0 * R1 * R2 * R3;
+    constexpr int m0 = n / R0, m1 = m0 / R1, m2 = m1 / R2;
+    constexpr int s1 = R0, s2 = R0 * R1, s3 = R0 * R1 * R2;
+    constexpr int NB0 = (E / R0) > 0 ? E / R0 : 1;
+    constexpr int NB1 = (E / R1) > 0 ? E / R1 : 1;
+    constexpr int NB2 = (E / R2) > 0 ? E / R2 : 1;
+    constexpr int NB3 = (E / R3) > 0 ? E / R3 : 1;
+    constexpr int kIdle = 1 << 28;
+    extern __shared__ float2 smem[];
+    const int G = a.G;
+    const int g = threadIdx.x / T;
+    const int lt0 = threadIdx.x - g * T;
+    const bool in_tile = g < G;
+    const TileContig tl{smem + (in_tile ? g : 0) * lds_len(n)};
+
+    const long long tr = (long long)blockIdx.x * G + g;
+    const bool ok = in_tile && tr < a.ntrans;
+    const int lt = ok ? lt0 : kIdle;
+    const long long t0 = ok ? tr : 0;
+    const long long c = t0 / a.nblocks;
+    const long long i = t0 - c * a.nblocks;
+    const long long s0 = i * a.b - a.skip;
+    const LdFrame ld{a.in + c * a.in_st, s0, a.lo, a.n, n, a.tmod};
+    const StFrame st{a.out + c * a.out_st, s0, a.n, n, a.skip, a.tmod};
+    const BotConv bot{(const float2*)a.kern, false, true};
+    const float2* tw0 = a.c.p[0].tw;
+    const float2* tw1 = a.c.p[1].tw;
+    const float2* tw2 = a.c.p[2].tw;
+
+    pass_first<R0, NB0>(tl, lt, T, m0, tw0, ld);
+    __syncthreads();
+    dif_mid<R1, NB1>(tl, lt, T, m1, s1, tw1);
+    __syncthreads();
+    dif_mid<R2, NB2>(tl, lt, T, m2, s2, tw2);
+    __syncthreads();
+    dif_bottom<R3, NB3>(tl, lt, T, s3, bot);
+    __syncthreads();
+    dit_mid<R2, NB2>(tl, lt, T, m2, s2, tw2);
+    __syncthreads();
+    dit_mid<R1, NB1>(tl, lt, T, m1, s1, tw1);
+    __syncthreads();
+    dit_last<R0, NB0>(tl, lt, T, m0, tw0, st);
+}
+
+static bool chain_is4(const ChainD& c, int r0, int r1, int r2, int r3)
+{
+    return c.np == 4 && c.p[0].R == r0 && c.p[1].R == r1 && c.p[2].R == r2 && c.p[3].R == r3;
+}
+
 hipError_t launch_block(const BlockArgs& a, const ShapeD& sh, hipStream_t st)
 {
     if (sh.E != 16)
@@ -88,10 +145,16 @@ hipError_t launch_block(const BlockArgs& a, const ShapeD& sh, hipStream_t st)
         return hipSuccess;
     if (tiles > 0x7fffffffLL)
         return hipErrorInvalidValue;
+    auto fn = k_block<16>;
+    // FREQOFFSETREAL 8192 = [16,8,8,8] (the tuned chain) has a specialised
+    // kernel; FFTSET_AMD_GENERIC=1 (read per call) keeps the generic one
+    const char* gen = getenv("FFTSET_AMD_GENERIC");
+    if (!(gen && atoi(gen) > 0) && a.L == 8192 && sh.T == 512 && chain_is4(a.c, 16, 8, 8, 8))
+        fn = k_block_fix<16, 8, 8, 8, 512>;
     if (sh.lds_bytes > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_block<16>, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
     (void)hipGetLastError();  // report this launch, not an earlier failed call
-    hipLaunchKernelGGL(k_block<16>, dim3((unsigned)tiles), dim3(sh.threads), sh.lds_bytes, st, a);
+    hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(sh.threads), sh.lds_bytes, st, a);
     return hipGetLastError();
 }
 

@@ -128,3 +128,40 @@ def test_blockconv_rejects_bad_use(fs):
     with pytest.raises(fftset_amd.FFTSetError):
         bc.run(x, x, flags=8)
     bc.destroy()
+
+
+def test_blockconv_specialised_chain_matches_generic(monkeypatch):
+    # with the tuned table, 1024 taps give L = 8192 on the chain [16,8,8,8],
+    # which runs a compile-time specialised frame kernel; it must agree with
+    # the generic runtime-chain kernel and with FP64 direct convolution
+    import os
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    tune = os.path.join(os.path.dirname(fftset_amd.__file__), "tune_mi355x.txt")
+    s = fftset_amd.FFTSet(tune_file=tune)
+    k = 1024
+    h = _taps(k, 19)
+    bc = s.blockconv(torch.from_numpy(h).cuda(), 0)
+    L = bc.complex_bins
+    assert L == 8192
+    assert list(s.create_fft(fftset_amd.FREQ_OFFSET_REAL, L).radices()) == [16, 8, 8, 8]
+    C, n = 3, 3 * bc.block_size + 321
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal((C, n)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+
+    def run():
+        y = torch.full((C, n), float("nan"), device="cuda")
+        bc.run(y, xd)
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    fixed = run()
+    monkeypatch.setenv("FFTSET_AMD_GENERIC", "1")
+    generic = run()
+    assert np.all(np.isfinite(fixed))
+    for c in range(C):
+        assert fp64.rel_rms(fixed[c], generic[c]) <= 1e-6, c
+        assert fp64.rel_rms(fixed[c], _fir64(x[c], h)) <= 1e-6 * np.log2(2 * L), c
+    bc.destroy()
+    s.destroy()
