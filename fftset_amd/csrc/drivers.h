@@ -280,4 +280,66 @@ __device__ __forceinline__ void run_stockham_to_tile(const ChainD& c, const Tile
     }
 }
 
+// ------------------------------------------- two-pass chains, compile time
+// A chain [R0, R1] fixed at compile time (16 elements per lane): the same
+// passes, twiddle tables and data order as the runtime-chain drivers above,
+// with constant radices, strides and lane counts.
+template <int R0, int R1>
+struct Fix2 {
+    static_assert(R0 > 0 && R1 > 0, "a two-pass chain");
+    static constexpr int n = R0 * R1;
+    static constexpr int m0 = n / R0, s1 = R0;
+    static constexpr int NB0 = (16 / R0) > 0 ? 16 / R0 : 1;
+    static constexpr int NB1 = (16 / R1) > 0 ? 16 / R1 : 1;
+    static constexpr int T0 = (m0 + NB0 - 1) / NB0, T1 = (s1 + NB1 - 1) / NB1;
+    static constexpr int T = T0 > T1 ? T0 : T1;  // lanes per transform
+
+    // true when a runtime chain and lane count are exactly this one
+    static bool matches(const ChainD& c, int lanes)
+    {
+        return c.np == 2 && c.p[0].R == R0 && c.p[1].R == R1 && c.p[0].m == m0 && c.p[1].m == 1 &&
+               c.p[1].s == s1 && lanes == T;
+    }
+};
+
+template <int R0, int R1, class Tile, class Load, class Store>
+__device__ __forceinline__ void run_stockham_fix2(const ChainD& c, const Tile& tl, int lt, const Load& ld,
+                                                  const Store& st)
+{
+    using F = Fix2<R0, R1>;
+    pass_first<R0, F::NB0>(tl, lt, F::T, F::m0, c.p[0].tw, ld);
+    __syncthreads();
+    stock_last<R1, F::NB1>(tl, lt, F::T, F::s1, st);
+}
+
+template <int R0, int R1, class Tile, class Load>
+__device__ __forceinline__ void run_stockham_to_tile_fix2(const ChainD& c, const Tile& tl, int lt, const Load& ld)
+{
+    using F = Fix2<R0, R1>;
+    pass_first<R0, F::NB0>(tl, lt, F::T, F::m0, c.p[0].tw, ld);
+    __syncthreads();
+    stock_mid<R1, F::NB1>(tl, lt, F::T, 1, F::s1, c.p[1].tw);
+}
+
+template <int R0, int R1, class Tile, class Load, class Store>
+__device__ __forceinline__ void run_conv_fix2(const ChainD& c, const Tile& tl, int lt, const Load& ld,
+                                              const Store& st, const BotConv& bot)
+{
+    using F = Fix2<R0, R1>;
+    pass_first<R0, F::NB0>(tl, lt, F::T, F::m0, c.p[0].tw, ld);
+    __syncthreads();
+    dif_bottom<R1, F::NB1>(tl, lt, F::T, F::s1, bot);
+    __syncthreads();
+    dit_last<R0, F::NB0>(tl, lt, F::T, F::m0, c.p[0].tw, st);
+}
+
+template <int R0, int R1, class Tile, class Load>
+__device__ __forceinline__ void run_kern_fix2(const ChainD& c, const Tile& tl, int lt, const Load& ld, float2* out)
+{
+    using F = Fix2<R0, R1>;
+    pass_first<R0, F::NB0>(tl, lt, F::T, F::m0, c.p[0].tw, ld);
+    __syncthreads();
+    dif_bottom<R1, F::NB1>(tl, lt, F::T, F::s1, BotKernOut{out});
+}
+
 }  // namespace fsa

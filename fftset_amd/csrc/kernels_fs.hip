@@ -10,6 +10,8 @@
 // DIF -> product -> DIT in place (the kernel spectrum is stored per row in
 // the same slot-major order) and a second column step finishes the
 // conjugate-data inverse.
+#include <cstdlib>
+
 #include "drivers.h"
 #include "kernels.h"
 
@@ -60,7 +62,8 @@ struct ColIo {
 };
 
 // Persistent column step: tile = (transform, group of C adjacent columns).
-template <int LK, int SK, int E, bool PERSIST>
+// R0 > 0: the column chain is [R0, R1], fixed at compile time.
+template <int LK, int SK, int E, bool PERSIST, int R0 = 0, int R1 = 0>
 __global__ void __launch_bounds__(1024) k_cols(const FsArgs a)
 {
     extern __shared__ float2 smem[];
@@ -76,8 +79,12 @@ __global__ void __launch_bounds__(1024) k_cols(const FsArgs a)
         const long long t = blockIdx.x / a.groups;
         const int n2 = (int)(blockIdx.x - t * a.groups) * C + g;
         const bool ok = in_col && n2 < a.B && t < a.ntrans;
-        run_stockham<E>(a.c, tl, ok ? lt0 : kIdle, T, Io::ld(a, ok ? t : 0, ok ? n2 : 0),
-                        Io::st(a, ok ? t : 0, ok ? n2 : 0));
+        if constexpr (R0 > 0)
+            run_stockham_fix2<R0, R1>(a.c, tl, ok ? lt0 : kIdle, Io::ld(a, ok ? t : 0, ok ? n2 : 0),
+                                      Io::st(a, ok ? t : 0, ok ? n2 : 0));
+        else
+            run_stockham<E>(a.c, tl, ok ? lt0 : kIdle, T, Io::ld(a, ok ? t : 0, ok ? n2 : 0),
+                            Io::st(a, ok ? t : 0, ok ? n2 : 0));
         return;
     }
 
@@ -126,12 +133,16 @@ __global__ void __launch_bounds__(1024) k_cols(const FsArgs a)
 }
 
 // Persistent row step: tile = (transform, group of C rows).
-template <int KIND, int SK, int E, bool PERSIST>
+// R0 > 0: the row chain is [R0, R1], fixed at compile time.
+template <int KIND, int SK, int E, bool PERSIST, int R0 = 0, int R1 = 0>
 __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
 {
     extern __shared__ float2 smem[];
     constexpr int kIdle = 1 << 28;
-    const int T = a.T, C = a.C;
+    int T = a.T;
+    if constexpr (R0 > 0)
+        T = Fix2<R0, R1>::T;
+    const int C = a.C;
     const int g = threadIdx.x / T;
     const int lt0 = threadIdx.x - g * T;
     const int LS = lds_len(a.B);
@@ -148,7 +159,10 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
         const LdCx<false> ld{(const float2*)fs_in(a, t0) + (long long)k1c * a.B};
         float2* row_out = (float2*)fs_out(a, t0) + (long long)k1c * a.B;
         if constexpr (KIND == RK_STOCK) {
-            run_stockham_to_tile<E>(a.c, tl, lt, T, ld);
+            if constexpr (R0 > 0)
+                run_stockham_to_tile_fix2<R0, R1>(a.c, tl, lt, ld);
+            else
+                run_stockham_to_tile<E>(a.c, tl, lt, T, ld);
             __syncthreads();
             const int nrows = min(C, a.A - k1_0);
             if (t >= a.ntrans || nrows <= 0)
@@ -170,9 +184,16 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
             }
         } else if constexpr (KIND == RK_CONV) {
             const BotConv bot{(const float2*)fs_kern(a, t0) + (long long)k1c * a.B, a.conj_kern != 0, fs_kern_shared(a)};
-            run_conv<E>(a.c, tl, lt, T, ld, StRowTw{row_out, a.tfs + (long long)k1c * a.B}, bot);
+            const StRowTw st{row_out, a.tfs + (long long)k1c * a.B};
+            if constexpr (R0 > 0)
+                run_conv_fix2<R0, R1>(a.c, tl, lt, ld, st, bot);
+            else
+                run_conv<E>(a.c, tl, lt, T, ld, st, bot);
         } else {
-            run_kern<E>(a.c, tl, lt, T, ld, row_out);
+            if constexpr (R0 > 0)
+                run_kern_fix2<R0, R1>(a.c, tl, lt, ld, row_out);
+            else
+                run_kern<E>(a.c, tl, lt, T, ld, row_out);
         }
         return;
     }
@@ -288,22 +309,47 @@ static hipError_t do_launch(F fn, F fn_persist, const FsArgs& a, const ShapeD& s
     return hipGetLastError();
 }
 
+// Chains with a compile-time form (the 240 x 256 split of 61440: columns
+// [16,15], rows [16,16]) use it unless FFTSET_AMD_GENERIC=1 (read per call)
+// asks for the runtime-chain kernels.
+static bool generic_forced()
+{
+    const char* g = getenv("FFTSET_AMD_GENERIC");
+    return g && atoi(g) > 0;
+}
+
+template <int LK, int SK>
+static hipError_t cols(const FsArgs& a, const ShapeD& sh, hipStream_t st)
+{
+    if (!persistent_mode() && !generic_forced() && Fix2<16, 15>::matches(a.c, a.T))
+        return do_launch(k_cols<LK, SK, 16, false, 16, 15>, k_cols<LK, SK, 16, true>, a, sh, st);
+    return do_launch(k_cols<LK, SK, 16, false>, k_cols<LK, SK, 16, true>, a, sh, st);
+}
+
+template <int KIND, int SK>
+static hipError_t rows(const FsArgs& a, const ShapeD& sh, hipStream_t st)
+{
+    if (!persistent_mode() && !generic_forced() && Fix2<16, 16>::matches(a.c, a.T))
+        return do_launch(k_rows<KIND, SK, 16, false, 16, 16>, k_rows<KIND, SK, 16, true>, a, sh, st);
+    return do_launch(k_rows<KIND, SK, 16, false>, k_rows<KIND, SK, 16, true>, a, sh, st);
+}
+
 hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st)
 {
     if (sh.E != 16)
         return hipErrorInvalidValue;
     if (store == CS_WORKTW) {
         switch (load) {
-        case CL_FOMOD: return do_launch(k_cols<CL_FOMOD, CS_WORKTW, 16, false>, k_cols<CL_FOMOD, CS_WORKTW, 16, true>, a, sh, st);
-        case CL_FOBINS: return do_launch(k_cols<CL_FOBINS, CS_WORKTW, 16, false>, k_cols<CL_FOBINS, CS_WORKTW, 16, true>, a, sh, st);
-        case CL_CX: return do_launch(k_cols<CL_CX, CS_WORKTW, 16, false>, k_cols<CL_CX, CS_WORKTW, 16, true>, a, sh, st);
-        case CL_CXCONJ: return do_launch(k_cols<CL_CXCONJ, CS_WORKTW, 16, false>, k_cols<CL_CXCONJ, CS_WORKTW, 16, true>, a, sh, st);
+        case CL_FOMOD: return cols<CL_FOMOD, CS_WORKTW>(a, sh, st);
+        case CL_FOBINS: return cols<CL_FOBINS, CS_WORKTW>(a, sh, st);
+        case CL_CX: return cols<CL_CX, CS_WORKTW>(a, sh, st);
+        case CL_CXCONJ: return cols<CL_CXCONJ, CS_WORKTW>(a, sh, st);
         }
     } else if (load == CL_CX) {
         if (store == CS_FODEMOD)
-            return do_launch(k_cols<CL_CX, CS_FODEMOD, 16, false>, k_cols<CL_CX, CS_FODEMOD, 16, true>, a, sh, st);
+            return cols<CL_CX, CS_FODEMOD>(a, sh, st);
         if (store == CS_CXCONJ)
-            return do_launch(k_cols<CL_CX, CS_CXCONJ, 16, false>, k_cols<CL_CX, CS_CXCONJ, 16, true>, a, sh, st);
+            return cols<CL_CX, CS_CXCONJ>(a, sh, st);
     }
     return hipErrorInvalidValue;
 }
@@ -314,17 +360,17 @@ hipError_t launch_rows(int kind, int store, const FsArgs& a, const ShapeD& sh, h
         return hipErrorInvalidValue;
     if (kind == RK_STOCK) {
         switch (store) {
-        case RS_FOBINS: return do_launch(k_rows<RK_STOCK, RS_FOBINS, 16, false>, k_rows<RK_STOCK, RS_FOBINS, 16, true>, a, sh, st);
-        case RS_CX: return do_launch(k_rows<RK_STOCK, RS_CX, 16, false>, k_rows<RK_STOCK, RS_CX, 16, true>, a, sh, st);
-        case RS_FODEMOD: return do_launch(k_rows<RK_STOCK, RS_FODEMOD, 16, false>, k_rows<RK_STOCK, RS_FODEMOD, 16, true>, a, sh, st);
-        case RS_CXCONJ: return do_launch(k_rows<RK_STOCK, RS_CXCONJ, 16, false>, k_rows<RK_STOCK, RS_CXCONJ, 16, true>, a, sh, st);
+        case RS_FOBINS: return rows<RK_STOCK, RS_FOBINS>(a, sh, st);
+        case RS_CX: return rows<RK_STOCK, RS_CX>(a, sh, st);
+        case RS_FODEMOD: return rows<RK_STOCK, RS_FODEMOD>(a, sh, st);
+        case RS_CXCONJ: return rows<RK_STOCK, RS_CXCONJ>(a, sh, st);
         }
         return hipErrorInvalidValue;
     }
     if (kind == RK_CONV)
-        return do_launch(k_rows<RK_CONV, RS_NONE, 16, false>, k_rows<RK_CONV, RS_NONE, 16, true>, a, sh, st);
+        return rows<RK_CONV, RS_NONE>(a, sh, st);
     if (kind == RK_KERN)
-        return do_launch(k_rows<RK_KERN, RS_NONE, 16, false>, k_rows<RK_KERN, RS_NONE, 16, true>, a, sh, st);
+        return rows<RK_KERN, RS_NONE>(a, sh, st);
     return hipErrorInvalidValue;
 }
 

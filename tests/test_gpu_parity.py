@@ -289,10 +289,14 @@ def test_allocation_failure_is_reported(fs):
 
 
 @pytest.mark.parametrize("mod,L,op", [(CX, 3840, "fwd"), (CX, 3840, "inv"), (FO, 4096, "conv"), (FO, 4096, "kern"),
-                                      (FO, 2048, "conv"), (FO, 2048, "grad"), (FO, 2048, "kern")])
+                                      (FO, 2048, "conv"), (FO, 2048, "grad"), (FO, 2048, "kern"),
+                                      (CX, 61440, "fwd"), (CX, 61440, "inv"), (FO, 61440, "fwd"),
+                                      (FO, 61440, "inv"), (FO, 61440, "conv"), (CX, 61440, "grad"),
+                                      (FO, 61440, "kern")])
 def test_specialised_kernels_match_generic(fs, monkeypatch, mod, L, op):
-    # compile-time specialised chains (kernels_fixed.hip) against the
-    # generic runtime-chain kernel on the same plan and inputs
+    # compile-time specialised chains (kernels_fixed.hip; the four-step
+    # kernels' [16,15] x [16,16] split of 61440) against the generic
+    # runtime-chain kernels on the same plan and inputs
     import torch
     f = fs.create_fft(mod, L)
     n = 64
