@@ -89,6 +89,7 @@ struct fftset_batch {
     hipStream_t cap_stream = nullptr;
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t mid = nullptr;  // after the largest four-step class's column step
 };
 
 namespace fsa {
@@ -490,13 +491,28 @@ static int launch_fs_step(const fftset_batch* b, int q, hipStream_t st)
     return FFTSET_OK;
 }
 
+// Four-step classes, largest (algorithmic bytes) first.
+static std::vector<const BatchClass*> four_step_classes(const fftset_batch* b)
+{
+    std::vector<const BatchClass*> fsc;
+    for (const BatchClass& k : b->classes)
+        if (k.four_step)
+            fsc.push_back(&k);
+    std::stable_sort(fsc.begin(), fsc.end(), [](const BatchClass* x, const BatchClass* y) {
+        return op_bytes(x->op, x->rc.L) * (long long)x->jobs.size() >
+               op_bytes(y->op, y->rc.L) * (long long)y->jobs.size();
+    });
+    return fsc;
+}
+
 // Launch schedules (the graph records the same structure):
-//  BRANCHES (default): each workgroup size of LDS tiles on a side branch
-//    forked at the start and joined at the end; each four-step class's chain
-//    (columns -> rows -> closing columns) on a branch of its own, the largest
-//    on the main stream, the others after the LDS groups on the side branches
-//    (with FFTSET_BATCH_FOUR_STEP_TILES: one chain of tile launches on the
-//    main stream);
+//  BRANCHES (default): the largest four-step class's chain (columns ->
+//    rows -> closing columns) on the main stream, the other four-step
+//    classes' chains on side branches from the start, and the LDS groups on
+//    side branches once the main chain's column step is done, so they share
+//    the GPU with the row and closing-column steps rather than with the
+//    column steps. With FFTSET_BATCH_FOUR_STEP_TILES: the LDS groups on side
+//    branches from the start, one chain of tile launches on the main stream;
 //  PHASED: the LDS groups and the four-step column step run concurrently,
 //    then the rows, then the closing columns;
 //  SERIAL: every group one after the other, largest workgroups first.
@@ -504,13 +520,6 @@ static int launch_fs_step(const fftset_batch* b, int q, hipStream_t st)
 static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
 {
     const int sched = branches ? b->sched : FFTSET_BATCH_SCHED_SERIAL;
-    auto fork_to = [&](int i) -> hipStream_t {
-        return hipStreamWaitEvent(b->side[i], b->fork, 0) == hipSuccess ? b->side[i] : nullptr;
-    };
-    auto join_from = [&](int i) {
-        return hipEventRecord(b->join[i], b->side[i]) == hipSuccess &&
-               hipStreamWaitEvent(st, b->join[i], 0) == hipSuccess;
-    };
     if (sched == FFTSET_BATCH_SCHED_SERIAL) {
         for (int i = (int)b->lds_r.size() - 1; i >= 0; --i)
             if (int rc = launch_range(b, b->d_tiles, b->lds_r[i], st))
@@ -520,64 +529,71 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
                 return rc;
         return FFTSET_OK;
     }
-    const int nside = std::min<int>((int)b->lds_r.size(), 3);
     if (hipEventRecord(b->fork, st) != hipSuccess)
         return FFTSET_E_DEVICE;
-    for (int i = 0; i < (int)b->lds_r.size(); ++i) {
-        hipStream_t s = i < nside ? fork_to(i) : st;
-        if (!s)
-            return FFTSET_E_DEVICE;
-        if (int rc = launch_range(b, b->d_tiles, b->lds_r[i], s))
+    bool forked[3] = {false, false, false};
+    auto side = [&](int i) -> hipStream_t {
+        if (!forked[i]) {
+            if (hipStreamWaitEvent(b->side[i], b->fork, 0) != hipSuccess)
+                return nullptr;
+            forked[i] = true;
+        }
+        return b->side[i];
+    };
+    auto join_all = [&]() {
+        for (int i = 0; i < 3; ++i)
+            if (forked[i] && (hipEventRecord(b->join[i], b->side[i]) != hipSuccess ||
+                              hipStreamWaitEvent(st, b->join[i], 0) != hipSuccess))
+                return FFTSET_E_DEVICE;
+        return FFTSET_OK;
+    };
+    auto lds_from = [&](int first_side, hipEvent_t after) {
+        for (int i = 0; i < (int)b->lds_r.size(); ++i) {
+            hipStream_t s = side((first_side + i) % 3);
+            if (!s || (after && hipStreamWaitEvent(s, after, 0) != hipSuccess))
+                return FFTSET_E_DEVICE;
+            if (int rc = launch_range(b, b->d_tiles, b->lds_r[i], s))
+                return rc;
+        }
+        return FFTSET_OK;
+    };
+    if (sched == FFTSET_BATCH_SCHED_PHASED || b->fs_tiles) {
+        if (int rc = lds_from(0, nullptr))
             return rc;
-    }
-    if (sched == FFTSET_BATCH_SCHED_PHASED) {
         if (int rc = launch_fs_step(b, 0, st))
             return rc;
-        for (int i = 0; i < nside; ++i)
-            if (!join_from(i))
-                return FFTSET_E_DEVICE;
+        if (sched == FFTSET_BATCH_SCHED_PHASED)
+            if (int rc = join_all())
+                return rc;
         for (int q = 1; q < 3; ++q)
             if (int rc = launch_fs_step(b, q, st))
                 return rc;
-        return FFTSET_OK;
+        return join_all();
     }
-    int used = nside;  // side streams forked so far
-    if (b->fs_tiles) {
-        for (int q = 0; q < 3; ++q)
-            if (int rc = launch_fs_step(b, q, st))
-                return rc;
-    } else {
-        // one chain per four-step class: the largest on the main stream, the
-        // others on the side branches, after the LDS group there
-        std::vector<const BatchClass*> fsc;
-        for (const BatchClass& k : b->classes)
-            if (k.four_step)
-                fsc.push_back(&k);
-        std::stable_sort(fsc.begin(), fsc.end(), [](const BatchClass* x, const BatchClass* y) {
-            return op_bytes(x->op, x->rc.L) * (long long)x->jobs.size() >
-                   op_bytes(y->op, y->rc.L) * (long long)y->jobs.size();
-        });
-        for (size_t c = 0; c < fsc.size(); ++c) {
-            hipStream_t s = st;
-            if (c > 0) {
-                const int i = (int)((c - 1) % 3);
-                if (i >= used) {
-                    for (int u = used; u <= i; ++u)
-                        if (!fork_to(u))
-                            return FFTSET_E_DEVICE;
-                    used = i + 1;
-                }
-                s = b->side[i];
-            }
-            for (int q = 0; q < 3; ++q)
-                if (int rc = launch_fs_class(*fsc[c], q, s))
-                    return rc;
-        }
-    }
-    for (int i = 0; i < used; ++i)
-        if (!join_from(i))
+    const std::vector<const BatchClass*> fsc = four_step_classes(b);
+    for (size_t c = 1; c < fsc.size(); ++c) {
+        hipStream_t s = side((int)((c - 1) % 3));
+        if (!s)
             return FFTSET_E_DEVICE;
-    return FFTSET_OK;
+        for (int q = 0; q < 3; ++q)
+            if (int rc = launch_fs_class(*fsc[c], q, s))
+                return rc;
+    }
+    hipEvent_t after = nullptr;
+    if (!fsc.empty()) {
+        if (int rc = launch_fs_class(*fsc[0], 0, st))
+            return rc;
+        if (hipEventRecord(b->mid, st) != hipSuccess)
+            return FFTSET_E_DEVICE;
+        after = b->mid;
+    }
+    if (int rc = lds_from(fsc.size() > 1 ? 1 : 0, after))
+        return rc;
+    if (!fsc.empty())
+        for (int q = 1; q < 3; ++q)
+            if (int rc = launch_fs_class(*fsc[0], q, st))
+                return rc;
+    return join_all();
 }
 
 }  // namespace fsa
@@ -650,7 +666,8 @@ int fftset_batch_commit(struct fftset_batch* b, unsigned flags)
                   hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
                   hipEventCreateWithFlags(&b->join[0], hipEventDisableTiming) == hipSuccess &&
                   hipEventCreateWithFlags(&b->join[1], hipEventDisableTiming) == hipSuccess &&
-                  hipEventCreateWithFlags(&b->join[2], hipEventDisableTiming) == hipSuccess;
+                  hipEventCreateWithFlags(&b->join[2], hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&b->mid, hipEventDisableTiming) == hipSuccess;
         if (!ok || hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             rc = FFTSET_E_DEVICE;
         } else {
@@ -801,6 +818,8 @@ void fftset_batch_destroy(struct fftset_batch* b)
     }
     if (b->fork)
         (void)hipEventDestroy(b->fork);
+    if (b->mid)
+        (void)hipEventDestroy(b->mid);
     if (b->dev_mem || b->work_pool || b->pass_work)
         (void)hipDeviceSynchronize();
     if (b->dev_mem)
