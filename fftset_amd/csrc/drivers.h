@@ -321,9 +321,9 @@ __device__ __forceinline__ void run_stockham_to_tile_fix2(const ChainD& c, const
     stock_mid<R1, F::NB1>(tl, lt, F::T, 1, F::s1, c.p[1].tw);
 }
 
-template <int R0, int R1, class Tile, class Load, class Store>
+template <int R0, int R1, class Tile, class Load, class Store, class Bottom>
 __device__ __forceinline__ void run_conv_fix2(const ChainD& c, const Tile& tl, int lt, const Load& ld,
-                                              const Store& st, const BotConv& bot)
+                                              const Store& st, const Bottom& bot)
 {
     using F = Fix2<R0, R1>;
     pass_first<R0, F::NB0>(tl, lt, F::T, F::m0, c.p[0].tw, ld);

@@ -288,6 +288,24 @@ struct BotConv {
     }
 };
 
+// BotConv with the adjoint and shared-spectrum choices fixed at compile time.
+template <bool CONJ, bool SHARED>
+struct BotConvC {
+    const float2* __restrict__ kern;
+    template <int R, class Tile>
+    __device__ __forceinline__ void apply(const Tile& tl, int b, int s, float2* v) const
+    {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float2 K = SHARED ? kern[k * s + b] : ldg(kern + k * s + b);
+            v[k] = cconj(CONJ ? cmulc(v[k], K) : cmul(v[k], K));
+        }
+        DFT<R>::run(v);
+#pragma unroll
+        for (int t = 0; t < R; ++t) tl.at(b * R + t) = v[t];
+    }
+};
+
 // Kernel spectrum: written out slot-major.
 struct BotKernOut {
     float2* __restrict__ out;

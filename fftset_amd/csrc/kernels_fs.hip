@@ -133,8 +133,10 @@ __global__ void __launch_bounds__(1024) k_cols(const FsArgs a)
 }
 
 // Persistent row step: tile = (transform, group of C rows).
-// R0 > 0: the row chain is [R0, R1], fixed at compile time.
-template <int KIND, int SK, int E, bool PERSIST, int R0 = 0, int R1 = 0>
+// R0 > 0: the row chain is [R0, R1], fixed at compile time. BOT >= 0 (with
+// R0 > 0, convolution): the adjoint (bit 0) and shared-spectrum (bit 1)
+// choices of the bottom fixed at compile time too.
+template <int KIND, int SK, int E, bool PERSIST, int R0 = 0, int R1 = 0, int BOT = -1>
 __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
 {
     extern __shared__ float2 smem[];
@@ -185,7 +187,10 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
         } else if constexpr (KIND == RK_CONV) {
             const BotConv bot{(const float2*)fs_kern(a, t0) + (long long)k1c * a.B, a.conj_kern != 0, fs_kern_shared(a)};
             const StRowTw st{row_out, a.tfs + (long long)k1c * a.B};
-            if constexpr (R0 > 0)
+            if constexpr (R0 > 0 && BOT >= 0)
+                run_conv_fix2<R0, R1>(a.c, tl, lt, ld, st,
+                                      BotConvC<(BOT & 1) != 0, (BOT & 2) != 0>{bot.kern});
+            else if constexpr (R0 > 0)
                 run_conv_fix2<R0, R1>(a.c, tl, lt, ld, st, bot);
             else
                 run_conv<E>(a.c, tl, lt, T, ld, st, bot);
@@ -329,8 +334,9 @@ static hipError_t cols(const FsArgs& a, const ShapeD& sh, hipStream_t st)
 template <int KIND, int SK>
 static hipError_t rows(const FsArgs& a, const ShapeD& sh, hipStream_t st)
 {
-    if (!persistent_mode() && !generic_forced() && Fix2<16, 16>::matches(a.c, a.T))
-        return do_launch(k_rows<KIND, SK, 16, false, 16, 16>, k_rows<KIND, SK, 16, true>, a, sh, st);
+    if constexpr (KIND != RK_CONV)  // convolutions: see launch_rows
+        if (!persistent_mode() && !generic_forced() && Fix2<16, 16>::matches(a.c, a.T))
+            return do_launch(k_rows<KIND, SK, 16, false, 16, 16>, k_rows<KIND, SK, 16, true>, a, sh, st);
     return do_launch(k_rows<KIND, SK, 16, false>, k_rows<KIND, SK, 16, true>, a, sh, st);
 }
 
@@ -367,8 +373,19 @@ hipError_t launch_rows(int kind, int store, const FsArgs& a, const ShapeD& sh, h
         }
         return hipErrorInvalidValue;
     }
-    if (kind == RK_CONV)
+    if (kind == RK_CONV) {
+        if (!persistent_mode() && !generic_forced() && Fix2<16, 16>::matches(a.c, a.T)) {
+            const int bot = (a.conj_kern ? 1 : 0) | (!a.kern_p && a.kern_st == 0 ? 2 : 0);
+            auto fp = k_rows<RK_CONV, RS_NONE, 16, true>;
+            switch (bot) {
+            case 0: return do_launch(k_rows<RK_CONV, RS_NONE, 16, false, 16, 16, 0>, fp, a, sh, st);
+            case 1: return do_launch(k_rows<RK_CONV, RS_NONE, 16, false, 16, 16, 1>, fp, a, sh, st);
+            case 2: return do_launch(k_rows<RK_CONV, RS_NONE, 16, false, 16, 16, 2>, fp, a, sh, st);
+            default: return do_launch(k_rows<RK_CONV, RS_NONE, 16, false, 16, 16, 3>, fp, a, sh, st);
+            }
+        }
         return rows<RK_CONV, RS_NONE>(a, sh, st);
+    }
     if (kind == RK_KERN)
         return rows<RK_KERN, RS_NONE>(a, sh, st);
     return hipErrorInvalidValue;
