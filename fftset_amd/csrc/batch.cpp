@@ -509,9 +509,9 @@ static std::vector<const BatchClass*> four_step_classes(const fftset_batch* b)
 //  BRANCHES (default): the largest four-step class's chain (columns ->
 //    rows -> closing columns) on the main stream, the other four-step
 //    classes' chains on side branches from the start, and the LDS groups on
-//    side branches once the main chain's column step is done, so they share
-//    the GPU with the row and closing-column steps rather than with the
-//    column steps. With FFTSET_BATCH_FOUR_STEP_TILES: the LDS groups on side
+//    one more side branch (ascending workgroup size) once the main chain's
+//    column step is done, so they share the GPU with the row and
+//    closing-column steps rather than with the column steps. With FFTSET_BATCH_FOUR_STEP_TILES: the LDS groups on side
 //    branches from the start, one chain of tile launches on the main stream;
 //  PHASED: the LDS groups and the four-step column step run concurrently,
 //    then the rows, then the closing columns;
@@ -547,10 +547,12 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
                 return FFTSET_E_DEVICE;
         return FFTSET_OK;
     };
-    auto lds_from = [&](int first_side, hipEvent_t after) {
+    // LDS groups from side branch `first`: one branch each (spread), or all
+    // on that one branch in ascending workgroup size, after event `after`
+    auto lds_from = [&](int first, hipEvent_t after, bool spread) {
         for (int i = 0; i < (int)b->lds_r.size(); ++i) {
-            hipStream_t s = side((first_side + i) % 3);
-            if (!s || (after && hipStreamWaitEvent(s, after, 0) != hipSuccess))
+            hipStream_t s = side(spread ? (first + i) % 3 : first);
+            if (!s || ((spread || i == 0) && after && hipStreamWaitEvent(s, after, 0) != hipSuccess))
                 return FFTSET_E_DEVICE;
             if (int rc = launch_range(b, b->d_tiles, b->lds_r[i], s))
                 return rc;
@@ -558,7 +560,7 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
         return FFTSET_OK;
     };
     if (sched == FFTSET_BATCH_SCHED_PHASED || b->fs_tiles) {
-        if (int rc = lds_from(0, nullptr))
+        if (int rc = lds_from(0, nullptr, true))
             return rc;
         if (int rc = launch_fs_step(b, 0, st))
             return rc;
@@ -587,7 +589,7 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
             return FFTSET_E_DEVICE;
         after = b->mid;
     }
-    if (int rc = lds_from(fsc.size() > 1 ? 1 : 0, after))
+    if (int rc = lds_from(fsc.size() > 1 ? 1 : 0, after, false))
         return rc;
     if (!fsc.empty())
         for (int q = 1; q < 3; ++q)
