@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--transforms", type=int, default=20000, help="ragged: batch size")
     ap.add_argument("--threads", type=int, default=256, choices=[256, 512], help="ragged: lanes per workgroup")
     ap.add_argument("--no-graph", action="store_true", help="ragged: launch phases directly")
-    ap.add_argument("--one-tile-per-wg", action="store_true",
-                    help="ragged: one workgroup per tile instead of persistent resident grids")
+    ap.add_argument("--persistent", action="store_true",
+                    help="ragged: persistent resident grids taking LDS tiles from a counter, instead of one "
+                         "workgroup per tile")
     ap.add_argument("--sched", default="branches", choices=["branches", "phased", "serial"],
                     help="ragged: order of the tile groups inside the graph")
     ap.add_argument("--baselines", default="group,pass,transform",
@@ -341,7 +342,7 @@ def workload_ragged(torch, fs, rng_np, args):
     kb.launch()
     torch.cuda.synchronize()
     kb.destroy()
-    flags = (0 if args.one_tile_per_wg else fftset_amd.BATCH_PERSISTENT) | \
+    flags = (fftset_amd.BATCH_PERSISTENT if args.persistent else 0) | \
         (fftset_amd.BATCH_NO_GRAPH if args.no_graph else 0) | \
         (fftset_amd.BATCH_THREADS_512 if args.threads == 512 else 0) | \
         {"branches": 0, "phased": fftset_amd.BATCH_SCHED_PHASED, "serial": fftset_amd.BATCH_SCHED_SERIAL}[args.sched]
@@ -354,7 +355,7 @@ def workload_ragged(torch, fs, rng_np, args):
            "algorithmic_bytes": b.bytes, "tiles": info["tiles"], "classes": info["classes"],
            "four_step_classes": info["four_step_classes"], "lds_bytes": info["lds_bytes"],
            "threads": args.threads, "graph": not args.no_graph, "schedule": args.sched,
-           "persistent": not args.one_tile_per_wg,
+           "persistent": bool(args.persistent),
            "transforms_per_step": n,
            "nominal_flop_per_step": float(sum(nominal_flop(int(L), "conv" if c else "fwd")
                                               for L, c in zip(sizes, is_conv)))}
