@@ -511,8 +511,9 @@ static std::vector<const BatchClass*> four_step_classes(const fftset_batch* b)
 //    classes' chains on side branches from the start, and the LDS groups on
 //    one more side branch (ascending workgroup size) once the main chain's
 //    column step is done, so they share the GPU with the row and
-//    closing-column steps rather than with the column steps. With FFTSET_BATCH_FOUR_STEP_TILES: the LDS groups on side
-//    branches from the start, one chain of tile launches on the main stream;
+//    closing-column steps rather than with the column steps. With
+//    FFTSET_BATCH_FOUR_STEP_TILES: the LDS groups on side branches from the
+//    start, one chain of tile launches on the main stream;
 //  PHASED: the LDS groups and the four-step column step run concurrently,
 //    then the rows, then the closing columns;
 //  SERIAL: every group one after the other, largest workgroups first.
@@ -589,7 +590,8 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
             return FFTSET_E_DEVICE;
         after = b->mid;
     }
-    if (int rc = lds_from(fsc.size() > 1 ? 1 : 0, after, false))
+    // the side branch after the other chains' (shared only beyond three classes)
+    if (int rc = lds_from(std::min<int>(std::max<int>((int)fsc.size() - 1, 0), 2), after, false))
         return rc;
     if (!fsc.empty())
         for (int q = 1; q < 3; ++q)
