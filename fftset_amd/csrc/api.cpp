@@ -59,17 +59,17 @@ int use_device(const Ctx* ctx)
     return FFTSET_OK;
 }
 
-// Finds or creates the table (N, a0, rows, cols).
-const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st)
+// Finds or creates the table (N, a0, rs, rows, cols).
+const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st, int rs)
 {
     for (const Table& t : ctx->tables)
-        if (t.N == N && t.a0 == a0 && t.rows == rows && t.cols == cols)
+        if (t.N == N && t.a0 == a0 && t.rs == rs && t.rows == rows && t.cols == cols)
             return t.dev;
-    Table t{N, a0, rows, cols, nullptr};
+    Table t{N, a0, rs, rows, cols, nullptr};
     const size_t bytes = sizeof(float2) * (size_t)rows * (size_t)cols;
     if (hipMalloc((void**)&t.dev, bytes) != hipSuccess)
         return nullptr;
-    if (launch_twiddles(t.dev, N, a0, rows, cols, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    if (launch_twiddles(t.dev, N, a0, rs, rows, cols, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
         (void)hipFree(t.dev);
         return nullptr;
     }
@@ -268,6 +268,14 @@ int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A, int
     rc = attach_modulation(p, ctx, st);
     if (rc != FFTSET_OK)
         return rc;
+    if (p->mod == MOD_FO) {
+        const long long L = p->lfft;
+        const int A = (int)p->fs.A, B = (int)p->fs.B;
+        p->fs.tfs_fo = get_table(ctx, 4 * L, 1, A, B, st, 4);  // x = (1 + 4 k1) n2
+        p->fs.trow_fo = get_table(ctx, 4LL * A, 1, 1, A, st);  // x = n1
+        if (!p->fs.tfs_fo || !p->fs.trow_fo)
+            return FFTSET_E_NOMEM;
+    }
     p->engine = ENG_FOUR_STEP;
     return FFTSET_OK;
 }
@@ -293,6 +301,10 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
     base.ntrans = count;
     base.conj_kern = op == OP_GRAD_IN;
     FsArgs ca = base, ra = base;
+    if (fo) {  // column step of the forward direction: folded modulation
+        ca.tfs = fs.tfs_fo;
+        ca.trow = fs.trow_fo;
+    }
     ca.c = fs.colc;
     ca.C = fs.cols.G;
     ca.T = fs.cols.T;
@@ -312,12 +324,13 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
     case OP_FWD:
         io(ca, in, in_st, work, work_st);
         io(ra, work, work_st, out, out_st);
-        if (!ok(launch_cols(fo ? CL_FOMOD : CL_CX, CS_WORKTW, ca, fs.cols, st)))
+        if (!ok(launch_cols(fo ? CL_FOMODC : CL_CX, CS_WORKTW, ca, fs.cols, st)))
             return FFTSET_E_DEVICE;
         return ok(launch_rows(RK_STOCK, fo ? RS_FOBINS : RS_CX, ra, fs.rows, st)) ? FFTSET_OK : FFTSET_E_DEVICE;
     case OP_INV:
         io(ca, in, in_st, work, work_st);
         io(ra, work, work_st, out, out_st);
+        ca.tfs = fs.tfs;  // no modulation on the inverse's input side
         if (!ok(launch_cols(fo ? CL_FOBINS : CL_CXCONJ, CS_WORKTW, ca, fs.cols, st)))
             return FFTSET_E_DEVICE;
         return ok(launch_rows(RK_STOCK, fo ? RS_FODEMOD : RS_CXCONJ, ra, fs.rows, st)) ? FFTSET_OK
@@ -325,7 +338,7 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
     case OP_KERN:
         io(ca, in, in_st, out, out_st);
         io(ra, out, out_st, out, out_st);
-        if (!ok(launch_cols(fo ? CL_FOMOD : CL_CX, CS_WORKTW, ca, fs.cols, st)))
+        if (!ok(launch_cols(fo ? CL_FOMODC : CL_CX, CS_WORKTW, ca, fs.cols, st)))
             return FFTSET_E_DEVICE;
         return ok(launch_rows(RK_KERN, RS_NONE, ra, fs.rows, st)) ? FFTSET_OK : FFTSET_E_DEVICE;
     case OP_CONV:
@@ -336,7 +349,7 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
         ra.kern_st = kern_st;
         FsArgs cb = ca;
         io(cb, work, work_st, out, out_st);
-        if (!ok(launch_cols(fo ? CL_FOMOD : CL_CX, CS_WORKTW, ca, fs.cols, st)))
+        if (!ok(launch_cols(fo ? CL_FOMODC : CL_CX, CS_WORKTW, ca, fs.cols, st)))
             return FFTSET_E_DEVICE;
         if (!ok(launch_rows(RK_CONV, RS_NONE, ra, fs.rows, st)))
             return FFTSET_E_DEVICE;

@@ -453,7 +453,11 @@ static int launch_fs_class(const BatchClass& k, int q, hipStream_t st, long long
     if (q == 0) {
         a.in_p = r.in;
         a.out_p = r.op == OP_KERN ? r.out : r.work;
-        const int load = r.op == OP_INV ? (fo ? CL_FOBINS : CL_CXCONJ) : (fo ? CL_FOMOD : CL_CX);
+        const int load = r.op == OP_INV ? (fo ? CL_FOBINS : CL_CXCONJ) : (fo ? CL_FOMODC : CL_CX);
+        if (load == CL_FOMODC) {  // folded modulation, as run_four_step
+            a.tfs = fs.tfs_fo;
+            a.trow = fs.trow_fo;
+        }
         e = launch_cols(load, CS_WORKTW, a, sh, st);
     } else if (q == 1) {
         a.in_p = r.op == OP_KERN ? (const float* const*)r.out : (const float* const*)r.work;

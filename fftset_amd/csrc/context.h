@@ -24,7 +24,7 @@ namespace fsa {
 // src/fftset_vec.c:1064-1120).
 struct Table {
     long long N;
-    int a0, rows, cols;
+    int a0, rs, rows, cols;
     float2* dev;
 };
 
@@ -36,6 +36,12 @@ struct FourStep {
     ChainD colc, rowc;    // chains of the column (length A) and row (length B) transforms
     ShapeD cols, rows;    // launch shapes of the column and row kernels
     const float2* tfs;    // tfs[k1*B + n2] = W_n^{n2 k1}
+    // FREQOFFSETREAL: the modulation W_{4n}^{n1 B + n2} of the column step
+    // split into a row factor (input) and a column factor folded into the
+    // output twiddles: trow_fo[n1] = W_{4A}^{n1}, tfs_fo[k1*B + n2] =
+    // W_n^{n2 k1} W_{4n}^{n2} = W_{4n}^{n2 (4 k1 + 1)}
+    const float2* tfs_fo = nullptr;
+    const float2* trow_fo = nullptr;
 };
 
 // Autotuned choices for one (modulation, size): radix chain and packing of
@@ -95,7 +101,7 @@ namespace fsa {
 hipStream_t thread_stream(int device);
 int use_device(const Ctx* ctx);
 void set_error(int code);
-const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st);
+const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st, int rs = 1);
 bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st);
 // Four-step split of L whose column and row tiles fit max_threads lanes;
 // prefer_A > 0 forces that column length when it is valid.

@@ -405,6 +405,24 @@ struct LdCol {
     __device__ __forceinline__ float2 operator()(int n1) const { return ld(B * n1 + n2); }
 };
 
+// FREQOFFSETREAL column load with the modulation split (four-step column
+// step): element n1 of column n2 is z[e] = (x[e] - i x[L+e]) W_{4L}^e with
+// e = n1 B + n2 and W_{4L}^e = W_{4A}^{n1} W_{4L}^{n2}. The column factor is
+// constant over the column's DFT and goes into the output twiddles; only
+// trow[n1] = W_{4A}^{n1} is read here.
+struct LdColFoMod {
+    const float* __restrict__ x;
+    int L, B, n2;
+    const float2* __restrict__ trow;
+    __device__ __forceinline__ float2 operator()(int n1) const
+    {
+        const int e = B * n1 + n2;
+        const float re = ldg(x + e), im = ldg(x + L + e);
+        const float2 w = trow[n1];
+        return make_float2(re * w.x + im * w.y, re * w.y - im * w.x);
+    }
+};
+
 template <class St>
 struct StCol {
     St st;

@@ -22,7 +22,7 @@ struct LdsArgs {
 hipError_t launch_lds(int mod, int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st);
 
 // Four-step (n = A*B) steps for transforms larger than LDS.
-enum ColLoad : int { CL_FOMOD = 0, CL_FOBINS = 1, CL_CX = 2, CL_CXCONJ = 3 };
+enum ColLoad : int { CL_FOMOD = 0, CL_FOBINS = 1, CL_CX = 2, CL_CXCONJ = 3, CL_FOMODC = 4 };
 enum ColStore : int { CS_WORKTW = 0, CS_FODEMOD = 1, CS_CXCONJ = 2 };
 enum RowKind : int { RK_STOCK = 0, RK_CONV = 1, RK_KERN = 2 };
 enum RowStore : int { RS_FOBINS = 0, RS_CX = 1, RS_FODEMOD = 2, RS_CXCONJ = 3, RS_NONE = 4 };
@@ -46,6 +46,9 @@ struct FsArgs {
     const float* const* in_p;
     float* const* out_p;
     const float* const* kern_p;
+    // FREQOFFSETREAL column loads with folded modulation (CL_FOMODC):
+    // trow[n1] = W_{4A}^{n1}; the column factor W_{4L}^{n2} is in tfs
+    const float2* trow;
 };
 
 hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);
@@ -124,6 +127,6 @@ int persistent_waves();
 bool persistent_mode();
 
 // out[r*cols + c] = W_N^{((a0 + r) * c) mod N}, computed in FP64.
-hipError_t launch_twiddles(float2* out, long long N, int a0, int rows, int cols, hipStream_t st);
+hipError_t launch_twiddles(float2* out, long long N, int a0, int rs, int rows, int cols, hipStream_t st);
 
 }  // namespace fsa

@@ -42,6 +42,8 @@ struct ColIo {
         const float* in = fs_in(a, t);
         if constexpr (LK == CL_FOMOD)
             return LdCol<LdFoMod>{{in, a.L, a.tmod}, a.B, n2};
+        else if constexpr (LK == CL_FOMODC)
+            return LdColFoMod{in, a.L, a.B, n2, a.trow};
         else if constexpr (LK == CL_FOBINS)
             return LdCol<LdFoBins>{{(const float2*)in, a.L}, a.B, n2};
         else if constexpr (LK == CL_CX)
@@ -347,6 +349,7 @@ hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, h
     if (store == CS_WORKTW) {
         switch (load) {
         case CL_FOMOD: return cols<CL_FOMOD, CS_WORKTW>(a, sh, st);
+        case CL_FOMODC: return cols<CL_FOMODC, CS_WORKTW>(a, sh, st);
         case CL_FOBINS: return cols<CL_FOBINS, CS_WORKTW>(a, sh, st);
         case CL_CX: return cols<CL_CX, CS_WORKTW>(a, sh, st);
         case CL_CXCONJ: return cols<CL_CXCONJ, CS_WORKTW>(a, sh, st);

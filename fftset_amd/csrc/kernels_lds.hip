@@ -10,16 +10,16 @@ hipError_t launch_lds_fo(int op, const LdsArgs& a, const ShapeD& sh, hipStream_t
 hipError_t launch_lds_cx(int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st);
 
 // ---------------------------------------------------------------- twiddles
-// out[r*cols + c] = W_N^{x}, x = ((a0 + r) * c) mod N, W_N = exp(-2 pi i / N):
+// out[r*cols + c] = W_N^{x}, x = ((a0 + rs*r) * c) mod N, W_N = exp(-2 pi i / N):
 // evaluated in FP64 (exact integer reduction of the angle) and rounded once
 // to FP32. This is the only FP64 arithmetic in the library.
-__global__ void k_twiddle_fp64(float2* __restrict__ out, long long N, int a0, int rows, int cols)
+__global__ void k_twiddle_fp64(float2* __restrict__ out, long long N, int a0, int rs, int rows, int cols)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)rows * cols)
         return;
     const long long r = idx / cols, c = idx - r * cols;
-    long long x = ((a0 + r) % N) * (c % N) % N;
+    long long x = ((a0 + (long long)rs * r) % N) * (c % N) % N;
     if ((4 * x) % N == 0) {
         const int q = (int)((4 * x) / N);  // quarter turns are exact
         const float cr[4] = {1.f, 0.f, -1.f, 0.f};
@@ -34,13 +34,13 @@ __global__ void k_twiddle_fp64(float2* __restrict__ out, long long N, int a0, in
     out[idx] = make_float2((float)cs, (float)sn);
 }
 
-hipError_t launch_twiddles(float2* out, long long N, int a0, int rows, int cols, hipStream_t st)
+hipError_t launch_twiddles(float2* out, long long N, int a0, int rs, int rows, int cols, hipStream_t st)
 {
     const long long n = (long long)rows * cols;
     const int threads = 256;
     (void)hipGetLastError();  // report this launch, not an earlier failed call
     hipLaunchKernelGGL(k_twiddle_fp64, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads), 0, st, out, N,
-                       a0, rows, cols);
+                       a0, rs, rows, cols);
     return hipGetLastError();
 }
 
