@@ -81,6 +81,11 @@ def load_library():
     lib.fftset_fft_complex_bins.restype = u
     lib.fftset_fft_radices.argtypes = [vp, ctypes.POINTER(u), u]
     lib.fftset_fft_radices.restype = u
+    lib.fftset_kernel_select.argtypes = [u]
+    lib.fftset_kernel_select.restype = u
+    ull = ctypes.POINTER(ctypes.c_ulonglong)
+    lib.fftset_kernel_counts.argtypes = [ull, ull]
+    lib.fftset_kernel_counts.restype = None
     lib.fftset_batch_create.argtypes = [vp]
     lib.fftset_batch_create.restype = vp
     lib.fftset_batch_add.argtypes = [vp, vp, ctypes.c_int, fp, fp, fp]
@@ -148,6 +153,23 @@ def _check(what):
     code = load_library().fftset_last_error()
     if code != OK:
         raise FFTSetError(code, what)
+
+
+SELECT_GENERIC = 1
+SELECT_PERSISTENT = 2
+
+
+def kernel_select(flags):
+    """Sets the process-wide kernel selection (SELECT_* flags); returns the
+    previous flags."""
+    return int(load_library().fftset_kernel_select(flags))
+
+
+def kernel_counts():
+    """(specialised, generic) kernel launches since the previous call."""
+    s, g = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+    load_library().fftset_kernel_counts(ctypes.byref(s), ctypes.byref(g))
+    return s.value, g.value
 
 
 def recommend_conv_length(kernel_length, max_block_size):

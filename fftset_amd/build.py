@@ -7,6 +7,7 @@ Objects go to fftset_amd/_build/, the library to fftset_amd/libfftset_amd.so.
 import argparse
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -39,17 +40,30 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
-def headers():
-    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
-    hs.append(os.path.join(ROOT, "include", "fftset.h"))
-    return hs
+_INCLUDE = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.M)
 
 
-def _compile(src, force, newest_header):
+def _deps(path, seen=None):
+    """Local headers `path` includes, transitively (quoted includes found in
+    csrc/ or include/)."""
+    seen = set() if seen is None else seen
+    with open(path) as f:
+        text = f.read()
+    for name in _INCLUDE.findall(text):
+        for d in (CSRC, os.path.join(ROOT, "include")):
+            h = os.path.join(d, name)
+            if os.path.exists(h) and h not in seen:
+                seen.add(h)
+                _deps(h, seen)
+                break
+    return seen
+
+
+def _compile(src, force):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     if not force and os.path.exists(obj):
         t = os.path.getmtime(obj)
-        if t >= os.path.getmtime(src) and t >= newest_header:
+        if all(t >= os.path.getmtime(d) for d in [src] + sorted(_deps(src))):
             return obj, None
     cmd = [hipcc()] + COMMON + ["-x", "hip", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -61,11 +75,10 @@ def _compile(src, force, newest_header):
 def build(force=True, jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
-    newest = max(os.path.getmtime(h) for h in headers())
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
     objs, errors = [], []
     with cf.ThreadPoolExecutor(jobs) as ex:
-        for obj, err in ex.map(lambda s: _compile(s, force, newest), srcs):
+        for obj, err in ex.map(lambda s: _compile(s, force), srcs):
             objs.append(obj)
             if err:
                 errors.append(err)

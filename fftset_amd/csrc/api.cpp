@@ -94,31 +94,41 @@ bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st)
     return true;
 }
 
-// Work space of at least `floats` for calls on stream `st`; a buffer that
-// grows is freed only after the stream's earlier work has finished.
-static int ensure_scratch(Ctx* ctx, hipStream_t st, size_t floats, float** out)
+// Work space of at least `floats` for a call on stream `st`. On success
+// `lease` holds the stream's scratch lock: the caller queues its launches
+// before releasing it. A buffer that must grow is replaced and the old one
+// retired, not freed (kernels queued earlier on the stream may still use it;
+// fftset_destroy frees retired buffers). Growing inside graph capture is
+// refused, since the allocation would not be part of the graph.
+static int lease_scratch(Ctx* ctx, hipStream_t st, size_t floats, std::unique_lock<std::mutex>* lease, float** out)
 {
-    std::lock_guard<std::mutex> lk(ctx->mu);
     Ctx::Scratch* s = nullptr;
-    for (Ctx::Scratch& c : ctx->scratch)
-        if (c.stream == st)
-            s = &c;
-    if (!s) {
-        ctx->scratch.push_back({st, nullptr, 0});
-        s = &ctx->scratch.back();
-    }
-    if (s->floats < floats) {
-        if (s->buf) {
-            (void)hipStreamSynchronize(st);
-            (void)hipFree(s->buf);
-            s->buf = nullptr;
-            s->floats = 0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (auto& c : ctx->scratch)
+            if (c->stream == st)
+                s = c.get();
+        if (!s) {
+            ctx->scratch.push_back(std::make_unique<Ctx::Scratch>());
+            s = ctx->scratch.back().get();
+            s->stream = st;
         }
-        if (hipMalloc((void**)&s->buf, floats * sizeof(float)) != hipSuccess)
+    }
+    std::unique_lock<std::mutex> lk(s->m);
+    if (s->floats < floats) {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+            return FFTSET_E_INVALID;
+        float* buf = nullptr;
+        if (hipMalloc((void**)&buf, floats * sizeof(float)) != hipSuccess)
             return FFTSET_E_NOMEM;
+        if (s->buf)
+            s->retired.push_back(s->buf);
+        s->buf = buf;
         s->floats = floats;
     }
     *out = s->buf;
+    *lease = std::move(lk);
     return FFTSET_OK;
 }
 
@@ -366,8 +376,9 @@ static int run_op(const fftset_fft* p, int op, long long count, float* out, long
     if (p->engine == ENG_LDS)
         return run_lds(p, op, count, out, out_st, in, in_st, kern, kern_st, st);
     const long long len2 = 2LL * p->lfft;
+    std::unique_lock<std::mutex> lease;
     if (op != OP_KERN && !work) {
-        int rc = ensure_scratch(p->ctx, st, (size_t)(count * len2), &work);
+        int rc = lease_scratch(p->ctx, st, (size_t)(count * len2), &lease, &work);
         if (rc != FFTSET_OK)
             return rc;
         work_st = len2;
@@ -413,10 +424,11 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
         float* w = work;
         long long wst = work_st;
         float* w2 = nullptr;
+        std::unique_lock<std::mutex> lease;  // held until both steps are queued
         if (!w || four) {
             const size_t need = (size_t)(count * len2) * ((!w && four) ? 2 : 1);
             float* s = nullptr;
-            int rc = ensure_scratch(p->ctx, st, need, &s);
+            int rc = lease_scratch(p->ctx, st, need, &lease, &s);
             if (rc != FFTSET_OK)
                 return rc;
             if (!w) {
@@ -553,9 +565,12 @@ void fftset_destroy(struct fftset* fc)
     }
     for (Table& t : ctx->tables)
         (void)hipFree(t.dev);
-    for (Ctx::Scratch& c : ctx->scratch)
-        if (c.buf)
-            (void)hipFree(c.buf);
+    for (auto& c : ctx->scratch) {
+        if (c->buf)
+            (void)hipFree(c->buf);
+        for (float* r : c->retired)
+            (void)hipFree(r);
+    }
     if (ctx->stage)
         (void)hipFree(ctx->stage);
     delete ctx;

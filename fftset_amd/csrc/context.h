@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -62,16 +63,20 @@ struct Ctx {
     std::vector<Table> tables;
     size_t table_bytes = 0;
     fftset_fft* first = nullptr;  // plans sorted by descending size (src/fftset.c:97-129)
-    std::mutex mu;                // guards scratch
+    std::mutex mu;  // guards the scratch list
     // Library-owned work space for calls that pass no work buffer, one per
-    // stream: calls on one stream are ordered, calls on different streams
-    // never share a buffer.
+    // stream: calls on different streams never share a buffer. A call holds
+    // its stream's lock from taking the buffer until its launches are queued,
+    // so calls from several host threads on one stream do not interleave
+    // their steps over the same buffer.
     struct Scratch {
-        hipStream_t stream;
-        float* buf;
-        size_t floats;
+        hipStream_t stream = nullptr;
+        float* buf = nullptr;
+        size_t floats = 0;
+        std::vector<float*> retired;  // outgrown buffers, freed by fftset_destroy
+        std::mutex m;
     };
-    std::vector<Scratch> scratch;
+    std::vector<std::unique_ptr<Scratch>> scratch;
     std::mutex stage_mu;          // guards stage
     float* stage = nullptr;       // device copies of host buffers passed to the drop-in calls
     size_t stage_floats = 0;

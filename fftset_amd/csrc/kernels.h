@@ -124,7 +124,12 @@ hipError_t launch_taps(float* dst, const float* taps, int k, int n, float scale,
 // number of tiles.
 long long persistent_grid(const void* fn, int threads, int lds_bytes, long long tiles);
 int persistent_waves();
+// Kernel selection (select.cpp): FFTSET_SELECT_PERSISTENT and
+// FFTSET_SELECT_GENERIC of fftset_kernel_select, and a counter of launches
+// of specialised (compile-time chain) and generic (runtime chain) kernels.
 bool persistent_mode();
+bool generic_forced();
+void count_launch(bool specialised);
 
 // out[r*cols + c] = W_N^{((a0 + r) * c) mod N}, computed in FP64.
 hipError_t launch_twiddles(float2* out, long long N, int a0, int rs, int rows, int cols, hipStream_t st);

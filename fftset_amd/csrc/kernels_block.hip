@@ -147,10 +147,11 @@ hipError_t launch_block(const BlockArgs& a, const ShapeD& sh, hipStream_t st)
         return hipErrorInvalidValue;
     auto fn = k_block<16>;
     // FREQOFFSETREAL 8192 = [16,8,8,8] (the tuned chain) has a specialised
-    // kernel; FFTSET_AMD_GENERIC=1 (read per call) keeps the generic one
-    const char* gen = getenv("FFTSET_AMD_GENERIC");
-    if (!(gen && atoi(gen) > 0) && a.L == 8192 && sh.T == 512 && chain_is4(a.c, 16, 8, 8, 8))
+    // kernel; the kernel selection can keep the generic one
+    const bool fix = !generic_forced() && a.L == 8192 && sh.T == 512 && chain_is4(a.c, 16, 8, 8, 8);
+    if (fix)
         fn = k_block_fix<16, 8, 8, 8, 512>;
+    count_launch(fix);
     if (sh.lds_bytes > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
     (void)hipGetLastError();  // report this launch, not an earlier failed call

@@ -10,8 +10,6 @@
 // DIF -> product -> DIT in place (the kernel spectrum is stored per row in
 // the same slot-major order) and a second column step finishes the
 // conjugate-data inverse.
-#include <cstdlib>
-
 #include "drivers.h"
 #include "kernels.h"
 
@@ -317,18 +315,15 @@ static hipError_t do_launch(F fn, F fn_persist, const FsArgs& a, const ShapeD& s
 }
 
 // Chains with a compile-time form (the 240 x 256 split of 61440: columns
-// [16,15], rows [16,16]) use it unless FFTSET_AMD_GENERIC=1 (read per call)
-// asks for the runtime-chain kernels.
-static bool generic_forced()
-{
-    const char* g = getenv("FFTSET_AMD_GENERIC");
-    return g && atoi(g) > 0;
-}
+// [16,15], rows [16,16]) use it unless the kernel selection asks for the
+// runtime-chain kernels.
 
 template <int LK, int SK>
 static hipError_t cols(const FsArgs& a, const ShapeD& sh, hipStream_t st)
 {
-    if (!persistent_mode() && !generic_forced() && Fix2<16, 15>::matches(a.c, a.T))
+    const bool fix = !persistent_mode() && !generic_forced() && Fix2<16, 15>::matches(a.c, a.T);
+    count_launch(fix);
+    if (fix)
         return do_launch(k_cols<LK, SK, 16, false, 16, 15>, k_cols<LK, SK, 16, true>, a, sh, st);
     return do_launch(k_cols<LK, SK, 16, false>, k_cols<LK, SK, 16, true>, a, sh, st);
 }
@@ -336,9 +331,14 @@ static hipError_t cols(const FsArgs& a, const ShapeD& sh, hipStream_t st)
 template <int KIND, int SK>
 static hipError_t rows(const FsArgs& a, const ShapeD& sh, hipStream_t st)
 {
-    if constexpr (KIND != RK_CONV)  // convolutions: see launch_rows
-        if (!persistent_mode() && !generic_forced() && Fix2<16, 16>::matches(a.c, a.T))
+    if constexpr (KIND != RK_CONV) {  // convolutions: see launch_rows
+        const bool fix = !persistent_mode() && !generic_forced() && Fix2<16, 16>::matches(a.c, a.T);
+        count_launch(fix);
+        if (fix)
             return do_launch(k_rows<KIND, SK, 16, false, 16, 16>, k_rows<KIND, SK, 16, true>, a, sh, st);
+    } else {
+        count_launch(false);
+    }
     return do_launch(k_rows<KIND, SK, 16, false>, k_rows<KIND, SK, 16, true>, a, sh, st);
 }
 
@@ -378,6 +378,7 @@ hipError_t launch_rows(int kind, int store, const FsArgs& a, const ShapeD& sh, h
     }
     if (kind == RK_CONV) {
         if (!persistent_mode() && !generic_forced() && Fix2<16, 16>::matches(a.c, a.T)) {
+            count_launch(true);
             const int bot = (a.conj_kern ? 1 : 0) | (!a.kern_p && a.kern_st == 0 ? 2 : 0);
             auto fp = k_rows<RK_CONV, RS_NONE, 16, true>;
             switch (bot) {

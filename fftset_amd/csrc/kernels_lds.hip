@@ -62,18 +62,6 @@ long long persistent_grid(const void* fn, int threads, int lds_bytes, long long 
     return tiles < resident ? tiles : resident;
 }
 
-// Persistent launches (grid = resident set, register prefetch of the next
-// tile) are opt-in through FFTSET_AMD_PERSIST=1.
-bool persistent_mode()
-{
-    static int m = -1;
-    if (m < 0) {
-        const char* e = getenv("FFTSET_AMD_PERSIST");
-        m = (e && atoi(e) > 0) ? 1 : 0;
-    }
-    return m == 1;
-}
-
 // Resident-grid multiplier (waves of workgroups per launch); 1 = exactly the
 // resident set. Read once from FFTSET_AMD_GRID_WAVES.
 int persistent_waves()
@@ -90,15 +78,17 @@ int persistent_waves()
 hipError_t launch_lds_fixed(int mod, int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st);
 
 // Chains with a compile-time specialisation (kernels_fixed.hip) use it,
-// unless FFTSET_AMD_GENERIC=1 (read per call) asks for the generic kernel.
+// unless the kernel selection asks for the generic kernel.
 hipError_t launch_lds(int mod, int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st)
 {
-    const char* g = getenv("FFTSET_AMD_GENERIC");
-    if (!persistent_mode() && !(g && atoi(g) > 0)) {
+    if (!persistent_mode() && !generic_forced()) {
         const hipError_t e = launch_lds_fixed(mod, op, a, sh, st);
-        if (e != hipErrorNotSupported)
+        if (e != hipErrorNotSupported) {
+            count_launch(true);
             return e;
+        }
     }
+    count_launch(false);
     return mod == MOD_FO ? launch_lds_fo(op, a, sh, st) : launch_lds_cx(op, a, sh, st);
 }
 

@@ -3,7 +3,9 @@
  * Drop-in for the reference interface in fftset.h of templeblock/fftset:
  * same names, argument meaning and buffer rules, except that every buffer
  * argument is a DEVICE pointer (hipMalloc'd memory, or any memory the GPU can
- * address) aligned to at least 16 bytes. The drop-in execution calls run on a
+ * address) aligned to at least 8 bytes (one complex value; hipMalloc gives
+ * 256). The synchronous drop-in calls also accept host arrays and stage them
+ * through device memory. The drop-in execution calls run on a
  * per-thread HIP stream and return when their results are complete, as the
  * reference's CPU calls do. Extensions (stream-ordered variants, uniform and
  * ragged batches, convolution adjoints) follow the drop-in section and have
@@ -131,8 +133,13 @@ int fftset_last_error(void);
  *                     (the output gradient) with in = the signal x
  * work (may be NULL) provides count*work_stride floats of scratch for the
  * operations that need it; when NULL the library uses its own, one buffer
- * per stream (so calls on different streams never share it). stream is a
- * hipStream_t (NULL = the default stream). Returns FFTSET_OK or an error. */
+ * per stream (so calls on different streams never share it). A buffer that
+ * must grow is replaced, and the old one is freed only by fftset_destroy,
+ * so work=NULL calls on one stream from several host threads are safe;
+ * growing is refused (FFTSET_E_INVALID) while the stream is being captured
+ * into a graph (pass work there).
+ * stream is a hipStream_t (NULL = the default stream). Returns FFTSET_OK or
+ * an error. */
 int fftset_fft_execute(const struct fftset_fft *fft, int op, size_t count, float *out, size_t out_stride,
                        const float *in, size_t in_stride, const float *kern, size_t kern_stride, float *work,
                        size_t work_stride, void *stream);
@@ -254,6 +261,20 @@ unsigned fftset_blockconv_block_size(const struct fftset_blockconv *bc);
 unsigned fftset_blockconv_complex_bins(const struct fftset_blockconv *bc);
 
 void fftset_blockconv_destroy(struct fftset_blockconv *bc);
+
+/* Kernel selection, process-wide (for tests and comparisons). Chains with a
+ * compile-time specialisation run it by default; FFTSET_SELECT_GENERIC runs
+ * the runtime-chain kernels instead, FFTSET_SELECT_PERSISTENT resident-size
+ * grids with register prefetch. Initial value from the environment
+ * (FFTSET_AMD_GENERIC=1, FFTSET_AMD_PERSIST=1), read once at load. Returns
+ * the previous flags. */
+#define FFTSET_SELECT_GENERIC    1u
+#define FFTSET_SELECT_PERSISTENT 2u
+unsigned fftset_kernel_select(unsigned flags);
+/* Launches of specialised and of runtime-chain kernels issued by the
+ * drop-in and uniform-batch calls since the previous call (which resets
+ * both counts). */
+void fftset_kernel_counts(unsigned long long *specialised, unsigned long long *generic);
 
 /* Plan introspection (for tests, benchmarks and the autotuner). */
 unsigned fftset_fft_complex_bins(const struct fftset_fft *fft);

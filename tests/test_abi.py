@@ -47,6 +47,32 @@ def test_recommend_conv_length_matches_oracle(oracle):
             assert fftset_amd.recommend_conv_length(k, b) == oracle.recommend_conv_length(k, b), (k, b)
 
 
+def _is_2a3b(n):
+    while n % 2 == 0:
+        n //= 2
+    while n % 3 == 0:
+        n //= 3
+    return n == 1
+
+
+def test_recommend_conv_length_exhaustive(oracle):
+    # every kernel length up to 65536 against a sweep of block sizes, through
+    # the C ABI: bit-exact with the oracle (src/fftset.c:131-167) and the
+    # invariants the formula guarantees -- a multiple of 16 bins of the form
+    # 16 * 2^a 3^b whose real length 2r holds k + max(b, 8k) - 1 samples
+    lib = fftset_amd.load_library()
+    olib = oracle.lib
+    for b in (0, 1, 31, 32, 100, 1000, 4096, 8191, 65536, 1 << 20, 1 << 24):
+        prev = 0
+        for k in range(1, 65537):
+            r = lib.fftset_recommend_conv_length(k, b)
+            assert r == olib.oracle_recommend_conv_length(k, b), (k, b)
+            if r != prev:  # the invariants depend on r only through its value
+                assert r % 16 == 0 and _is_2a3b(r // 16), (k, b, r)
+                prev = r
+            assert 2 * r >= k + max(b, 8 * k) - 1, (k, b, r)
+
+
 def test_null_arguments_are_rejected_without_a_device():
     lib = fftset_amd.load_library()
     assert lib.fftset_init(None) != 0

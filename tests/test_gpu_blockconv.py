@@ -130,7 +130,7 @@ def test_blockconv_rejects_bad_use(fs):
     bc.destroy()
 
 
-def test_blockconv_specialised_chain_matches_generic(monkeypatch):
+def test_blockconv_specialised_chain_matches_generic():
     # with the tuned table, 1024 taps give L = 8192 on the chain [16,8,8,8],
     # which runs a compile-time specialised frame kernel; it must agree with
     # the generic runtime-chain kernel and with FP64 direct convolution
@@ -156,9 +156,15 @@ def test_blockconv_specialised_chain_matches_generic(monkeypatch):
         torch.cuda.synchronize()
         return y.cpu().numpy()
 
+    fftset_amd.kernel_counts()
     fixed = run()
-    monkeypatch.setenv("FFTSET_AMD_GENERIC", "1")
-    generic = run()
+    assert fftset_amd.kernel_counts() == (1, 0)  # the [16,8,8,8] frame kernel ran
+    prev = fftset_amd.kernel_select(fftset_amd.SELECT_GENERIC)
+    try:
+        generic = run()
+        assert fftset_amd.kernel_counts() == (0, 1)
+    finally:
+        fftset_amd.kernel_select(prev)
     assert np.all(np.isfinite(fixed))
     for c in range(C):
         assert fp64.rel_rms(fixed[c], generic[c]) <= 1e-6, c

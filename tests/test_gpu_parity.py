@@ -273,6 +273,71 @@ def test_drop_in_calls_from_concurrent_threads(fs, L):
         assert np.array_equal(host(y), ec)
 
 
+def test_scratch_growth_from_threads_on_one_stream():
+    # work-less four-step calls from several host threads on one stream, with
+    # growing batch sizes: a thread that grows the library's scratch must not
+    # free the buffer another thread was just handed (outgrown buffers are
+    # kept until fftset_destroy)
+    import threading
+    import torch
+    s = fftset_amd.FFTSet()
+    try:
+        L = 61440
+        f = s.create_fft(CX, L)
+        rng = np.random.default_rng(91)
+        sizes = [1, 2, 3, 4, 5, 6, 7, 8]
+        xs = [torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda() for n in sizes]
+        ys = [torch.empty_like(x) for x in xs]
+        stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        errors = []
+
+        def worker(i):
+            try:
+                for _ in range(3):
+                    f.execute(fftset_amd.OP_FORWARD, sizes[i], ys[i], xs[i], stream=stream.cuda_stream)
+            except Exception as e:  # pragma: no cover
+                errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(sizes))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        torch.cuda.synchronize()
+        assert not errors
+        for x, y in zip(xs, ys):
+            xh, yh = x.cpu().numpy(), y.cpu().numpy()
+            for r in (0, len(xh) - 1):
+                assert fp64.rel_rms(yh[r], fp64.cx_forward64(xh[r])) <= fp64_tol(L)
+    finally:
+        s.destroy()
+
+
+def test_scratch_growth_refused_under_capture():
+    # growing the library's scratch inside a graph capture is refused with
+    # FFTSET_E_INVALID (the allocation would not be part of the graph)
+    import torch
+    s = fftset_amd.FFTSet()
+    try:
+        L = 61440
+        f = s.create_fft(CX, L)
+        x = torch.zeros(2, 2 * L, device="cuda")
+        y = torch.empty_like(x)
+        stream = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        with pytest.raises(fftset_amd.FFTSetError) as e:
+            with torch.cuda.graph(g, stream=stream):
+                f.execute(fftset_amd.OP_FORWARD, 2, y, x, stream=stream.cuda_stream)
+        assert e.value.code == fftset_amd.E_INVALID
+        # outside capture the same call allocates and runs
+        f.execute(fftset_amd.OP_FORWARD, 2, y, x)
+        torch.cuda.synchronize()
+        assert np.all(y.cpu().numpy() == 0)
+    finally:
+        s.destroy()
+
+
 def test_allocation_failure_is_reported(fs):
     # a work-less four-step batch far beyond HBM: the scratch allocation fails
     # and is reported before any kernel is launched; the set stays usable
@@ -293,7 +358,7 @@ def test_allocation_failure_is_reported(fs):
                                       (CX, 61440, "fwd"), (CX, 61440, "inv"), (FO, 61440, "fwd"),
                                       (FO, 61440, "inv"), (FO, 61440, "conv"), (CX, 61440, "grad"),
                                       (FO, 61440, "kern")])
-def test_specialised_kernels_match_generic(fs, monkeypatch, mod, L, op):
+def test_specialised_kernels_match_generic(fs, mod, L, op):
     # compile-time specialised chains (kernels_fixed.hip; the four-step
     # kernels' [16,15] x [16,16] split of 61440) against the generic
     # runtime-chain kernels on the same plan and inputs
@@ -321,8 +386,17 @@ def test_specialised_kernels_match_generic(fs, monkeypatch, mod, L, op):
         torch.cuda.synchronize()
         return y.cpu().numpy()
 
+    fftset_amd.kernel_counts()
     fixed = run()
-    monkeypatch.setenv("FFTSET_AMD_GENERIC", "1")
-    generic = run()
+    spec, gen = fftset_amd.kernel_counts()
+    # the compile-time chain was chosen (at least one specialised launch)
+    assert spec >= 1, (spec, gen)
+    prev = fftset_amd.kernel_select(fftset_amd.SELECT_GENERIC)
+    try:
+        generic = run()
+        spec2, gen2 = fftset_amd.kernel_counts()
+    finally:
+        fftset_amd.kernel_select(prev)
+    assert spec2 == 0 and gen2 >= 1, (spec2, gen2)
     assert np.all(np.isfinite(fixed))
     assert np.max(fp64.rel_rms(fixed, generic)) <= 1e-6
