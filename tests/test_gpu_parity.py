@@ -34,6 +34,16 @@ def fs():
     s.destroy()
 
 
+@pytest.fixture(scope="module")
+def tuned_fs():
+    import os
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    s = fftset_amd.FFTSet(tune_file=os.path.join(os.path.dirname(fftset_amd.__file__), "tune_mi355x.txt"))
+    yield s
+    s.destroy()
+
+
 FO, CX = fftset_amd.FREQ_OFFSET_REAL, fftset_amd.COMPLEX
 
 
@@ -358,12 +368,16 @@ def test_allocation_failure_is_reported(fs):
                                       (CX, 61440, "fwd"), (CX, 61440, "inv"), (FO, 61440, "fwd"),
                                       (FO, 61440, "inv"), (FO, 61440, "conv"), (CX, 61440, "grad"),
                                       (FO, 61440, "kern")])
-def test_specialised_kernels_match_generic(fs, mod, L, op):
+def test_specialised_kernels_match_generic(tuned_fs, mod, L, op):
     # compile-time specialised chains (kernels_fixed.hip; the four-step
     # kernels' [16,15] x [16,16] split of 61440) against the generic
-    # runtime-chain kernels on the same plan and inputs
+    # runtime-chain kernels on the same plan and inputs. The specialised
+    # chains are the MI355X tune table's, so the plans come from a set that
+    # loaded it.
     import torch
-    f = fs.create_fft(mod, L)
+    f = tuned_fs.create_fft(mod, L)
+    expect = {3840: [15, 16, 16], 4096: [16, 16, 16], 2048: [8, 16, 16], 61440: [16, 15, 0, 16, 16]}[L]
+    assert f.radices() == expect
     n = 64
     rng = np.random.default_rng(L + len(op))
     x = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
