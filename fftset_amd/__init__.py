@@ -24,6 +24,8 @@ E_INVALID = -1
 E_NOMEM = -2
 E_DEVICE = -3
 E_UNSUPPORTED = -4
+E_TOO_LARGE = -5
+NOALLOC_WORK_FLOATS = 16 << 20
 
 OP_FORWARD = 0
 OP_INVERSE = 1
@@ -77,6 +79,8 @@ def load_library():
     lib.fftset_last_error.restype = ctypes.c_int
     lib.fftset_fft_execute.argtypes = [vp, ctypes.c_int, sz, fp, sz, fp, sz, fp, sz, fp, sz, vp]
     lib.fftset_fft_execute.restype = ctypes.c_int
+    lib.fftset_fft_execute_noalloc.argtypes = [vp, ctypes.c_int, sz, fp, sz, fp, sz, fp, sz, vp]
+    lib.fftset_fft_execute_noalloc.restype = ctypes.c_int
     lib.fftset_fft_complex_bins.argtypes = [vp]
     lib.fftset_fft_complex_bins.restype = u
     lib.fftset_fft_radices.argtypes = [vp, ctypes.POINTER(u), u]
@@ -250,6 +254,22 @@ class FFT:
             _ptr(work), work_stride, stream or None)
         if rc != OK:
             raise FFTSetError(rc, "execute")
+
+    def execute_noalloc(self, op, count, out, inp, kern=None, out_stride=None, in_stride=None, kern_stride=None,
+                        stream=None):
+        """fftset_fft_execute_noalloc: ``execute`` without a work buffer and
+        without any allocation (work space from the library's static device
+        array of NOALLOC_WORK_FLOATS floats)."""
+        n2 = 2 * self.complex_bins
+        out_stride = n2 if out_stride is None else out_stride
+        in_stride = n2 if in_stride is None else in_stride
+        if kern_stride is None:
+            kern_stride = n2 if kern is not None else 0
+        rc = load_library().fftset_fft_execute_noalloc(
+            self.handle, op, count, _ptr(out), out_stride, _ptr(inp), in_stride, _ptr(kern), kern_stride,
+            _stream_handle(stream))
+        if rc != OK:
+            raise FFTSetError(rc, "execute_noalloc")
 
 
 BATCH_NO_GRAPH = 1

@@ -388,7 +388,7 @@ static int run_op(const fftset_fft* p, int op, long long count, float* out, long
 
 int execute(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
             long long in_st, const float* kern, long long kern_st, float* work, long long work_st,
-            hipStream_t st)
+            hipStream_t st, float* work2)
 {
     if (!p || count < 0 || op < 0 || op > FFTSET_OP_CONV_GRAD_KERNEL)
         return FFTSET_E_INVALID;
@@ -423,9 +423,9 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
         const bool four = p->engine == ENG_FOUR_STEP;
         float* w = work;
         long long wst = work_st;
-        float* w2 = nullptr;
+        float* w2 = four ? work2 : nullptr;
         std::unique_lock<std::mutex> lease;  // held until both steps are queued
-        if (!w || four) {
+        if (!w || (four && !w2)) {
             const size_t need = (size_t)(count * len2) * ((!w && four) ? 2 : 1);
             float* s = nullptr;
             int rc = lease_scratch(p->ctx, st, need, &lease, &s);
@@ -547,6 +547,9 @@ int fftset_init(struct fftset* fc)
     }
     if (const char* tf = getenv("FFTSET_AMD_TUNE_FILE"))
         (void)load_tune_file(ctx, tf);
+    // the no-allocation entry point's work array on this device (a failed
+    // lookup disables only that entry point: it then reports FFTSET_E_DEVICE)
+    (void)noalloc_work(ctx->device, &ctx->noalloc);
     fc->impl = ctx;
     return 0;
 }

@@ -79,6 +79,7 @@ struct Ctx {
     std::vector<std::unique_ptr<Scratch>> scratch;
     std::mutex stage_mu;          // guards stage
     float* stage = nullptr;       // device copies of host buffers passed to the drop-in calls
+    float* noalloc = nullptr;     // the device's static work array (fftset_fft_execute_noalloc)
     size_t stage_floats = 0;
 };
 
@@ -121,8 +122,13 @@ const TuneEntry* find_tune(const Ctx* ctx, int mod, unsigned L);
 int load_tune_file(Ctx* ctx, const char* path);
 
 // Runs `op` over a uniform batch on `stream` (asynchronous). Returns an
-// FFTSET_* code.
+// FFTSET_* code. work2 (optional, count * 2L floats): the second work
+// region of a four-step CONV_GRAD_KERNEL, which otherwise comes from the
+// library's scratch.
 int execute(const fftset_fft* fft, int op, long long count, float* out, long long out_st, const float* in,
             long long in_st, const float* kern, long long kern_st, float* work, long long work_st,
-            hipStream_t stream);
+            hipStream_t stream, float* work2 = nullptr);
+// Device address of the static work array of fftset_fft_execute_noalloc on
+// `device` (which must be current); looked up once per device.
+int noalloc_work(int device, float** out);
 }  // namespace fsa

@@ -108,6 +108,8 @@ unsigned fftset_recommend_conv_length(unsigned kernel_length, unsigned max_block
 #define FFTSET_E_NOMEM         (-2)  /* host or device allocation failed      */
 #define FFTSET_E_DEVICE        (-3)  /* a HIP call failed                     */
 #define FFTSET_E_UNSUPPORTED   (-4)  /* size or operation not supported       */
+#define FFTSET_E_TOO_LARGE     (-5)  /* work space beyond the no-allocation limit
+                                        of fftset_fft_execute_noalloc          */
 
 /* Returns the code of the most recent failure on the calling thread and
  * clears it (FFTSET_OK when nothing failed since the last call). */
@@ -137,12 +139,30 @@ int fftset_last_error(void);
  * must grow is replaced, and the old one is freed only by fftset_destroy,
  * so work=NULL calls on one stream from several host threads are safe;
  * growing is refused (FFTSET_E_INVALID) while the stream is being captured
- * into a graph (pass work there).
+ * into a graph (pass work, or use fftset_fft_execute_noalloc, there).
  * stream is a hipStream_t (NULL = the default stream). Returns FFTSET_OK or
  * an error. */
 int fftset_fft_execute(const struct fftset_fft *fft, int op, size_t count, float *out, size_t out_stride,
                        const float *in, size_t in_stride, const float *kern, size_t kern_stride, float *work,
                        size_t work_stride, void *stream);
+
+/* fftset_fft_execute with work = NULL, for callers that cannot allocate
+ * device memory (code under graph capture, serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. It calls no device or
+ * host allocator: the work space that fftset_fft_execute would allocate
+ * comes from a static device array of FFTSET_NOALLOC_WORK_FLOATS floats
+ * (64 MiB, held by each GPU the library runs on until the process ends).
+ * Needs (floats): four-step plans (complex_bins > 16384), every operation
+ * but GET_KERNEL: count * 2 * complex_bins, CONV_GRAD_KERNEL twice that;
+ * LDS plans: CONV_GRAD_KERNEL count * 2 * complex_bins, others none.
+ * Larger calls fail with FFTSET_E_TOO_LARGE before any work is queued.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (different streams or host threads without ordering) race and
+ * give wrong results. Other arguments as fftset_fft_execute. */
+#define FFTSET_NOALLOC_WORK_FLOATS (16u << 20)
+int fftset_fft_execute_noalloc(const struct fftset_fft *fft, int op, size_t count, float *out, size_t out_stride,
+                               const float *in, size_t in_stride, const float *kern, size_t kern_stride,
+                               void *stream);
 
 /* Adjoints of fftset_fft_conv (synchronous, like the drop-in calls).
  * For FREQOFFSETREAL, with y = conv(x, h) = L * negacyclic(x, h) and an
