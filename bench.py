@@ -59,8 +59,8 @@ def parse():
                          "workgroup per tile")
     ap.add_argument("--sched", default="branches", choices=["branches", "phased", "serial"],
                     help="ragged: order of the tile groups inside the graph")
-    ap.add_argument("--baselines", default="group,pass,transform",
-                    help="ragged: comma list of baselines to time (group, pass, transform, none)")
+    ap.add_argument("--baselines", default="group,pass,reference,transform",
+                    help="ragged: comma list of baselines to time (group, pass, reference, transform, none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
@@ -401,11 +401,12 @@ def workload_ragged(torch, fs, rng_np, args):
 
     def baselines(torch, steps):
         out = {}
-        modes = [m for m in args.baselines.split(",") if m in ("group", "pass", "transform")]
+        modes = [m for m in args.baselines.split(",") if m in ("group", "pass", "reference", "transform")]
         for m in modes:
             mode = {"group": fftset_amd.BASELINE_PER_GROUP, "pass": fftset_amd.BASELINE_PER_PASS,
+                    "reference": fftset_amd.BASELINE_REFERENCE_PASSES,
                     "transform": fftset_amd.BASELINE_PER_TRANSFORM}[m]
-            reps = steps if m == "group" else max(1, min(3, steps))
+            reps = steps if m in ("group", "reference") else max(1, min(3, steps))
             b.launch_baseline(mode)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -415,7 +416,8 @@ def workload_ragged(torch, fs, rng_np, args):
             e1.record()
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1) / reps
-            out[f"per_{m}_launches"] = {"ms": round(ms, 4), "GB/s": round(b.bytes / ms / 1e6, 2)}
+            name = "per_reference_pass_launches" if m == "reference" else f"per_{m}_launches"
+            out[name] = {"ms": round(ms, 4), "GB/s": round(b.bytes / ms / 1e6, 2)}
         return out
 
     return kernels, bytes_per, cfg, check, cpu_sample, baselines
@@ -590,6 +592,12 @@ def main():
         out["per_size_launch_baselines"] = baselines(torch, args.steps)
         g = out["per_size_launch_baselines"]
         out["speedup_vs_baselines"] = {k: round(v["ms"] / step_ms, 3) for k, v in g.items()}
+        if "per_reference_pass_launches" in g:
+            # the >= 5x target is against the reference's own pass structure
+            # on the GPU (one launch per sweep of its first pass, its
+            # planner's chain at its vector width, product and last pass)
+            out["target_5x"] = {"baseline": "per_reference_pass_launches",
+                                "speedup": out["speedup_vs_baselines"]["per_reference_pass_launches"]}
     if not args.no_cpu and cpu_sample is not None:
         out["cpu_baseline"] = cpu_sample(args.cpu_seconds)
     fs.destroy()

@@ -85,6 +85,8 @@ def load_library():
     lib.fftset_fft_complex_bins.restype = u
     lib.fftset_fft_radices.argtypes = [vp, ctypes.POINTER(u), u]
     lib.fftset_fft_radices.restype = u
+    lib.fftset_baseline_chain.argtypes = [vp, u, ctypes.POINTER(u), u]
+    lib.fftset_baseline_chain.restype = u
     lib.fftset_kernel_select.argtypes = [u]
     lib.fftset_kernel_select.restype = u
     ull = ctypes.POINTER(ctypes.c_ulonglong)
@@ -174,6 +176,18 @@ def kernel_counts():
     s, g = ctypes.c_ulonglong(), ctypes.c_ulonglong()
     load_library().fftset_kernel_counts(ctypes.byref(s), ctypes.byref(g))
     return s.value, g.value
+
+
+def baseline_chain(modulation, complex_bins):
+    """The reference's pass structure that BASELINE_REFERENCE_PASSES runs:
+    (split radices of its first pass, its planner's inner chain)."""
+    buf = (ctypes.c_uint * 64)()
+    n = load_library().fftset_baseline_chain(_modulation(modulation), int(complex_bins), buf, 64)
+    if n == 0:
+        raise FFTSetError(E_UNSUPPORTED, f"baseline_chain({modulation}, {complex_bins})")
+    r = [buf[i] for i in range(n)]
+    z = r.index(0)
+    return r[:z], r[z + 1:]
 
 
 def recommend_conv_length(kernel_length, max_block_size):
@@ -281,6 +295,7 @@ BATCH_FOUR_STEP_TILES = 32
 BASELINE_PER_GROUP = 1
 BASELINE_PER_TRANSFORM = 2
 BASELINE_PER_PASS = 3
+BASELINE_REFERENCE_PASSES = 4
 BLOCKCONV_HISTORY = 1
 
 

@@ -21,6 +21,7 @@
 
 #include "context.h"
 #include "planner.h"
+#include "refpass.h"
 
 namespace fsa {
 
@@ -82,6 +83,7 @@ struct fftset_batch {
     std::vector<TileRange> group_ranges;
     std::vector<TileRange> job_ranges;
     float2* pass_work = nullptr;     // per-pass baseline work tiles (allocated on first use)
+    std::vector<RefPassClass> refpass;  // reference-pass-structure baseline (set up on first use)
     int lds_bytes = 0;  // largest of any range
     long long alg_bytes = 0;
     hipGraph_t graph = nullptr;
@@ -765,6 +767,25 @@ int fftset_batch_launch_baseline(struct fftset_batch* b, int mode, void* stream)
                          ? FFTSET_OK
                          : FFTSET_E_DEVICE;
         }
+    } else if (mode == FFTSET_BASELINE_REFERENCE_PASSES) {
+        if (b->refpass.empty()) {
+            for (const BatchClass& k : b->classes) {
+                RefPassClass r;
+                rc = refpass_setup(b->ctx, k.p, k.op, k.rc, (long long)k.jobs.size(), &r, st);
+                if (rc != FFTSET_OK)
+                    break;
+                b->refpass.push_back(r);
+            }
+            if (rc != FFTSET_OK) {
+                for (RefPassClass& r : b->refpass) refpass_free(&r);
+                b->refpass.clear();
+                set_error(rc);
+                return rc;
+            }
+        }
+        for (const RefPassClass& r : b->refpass)
+            if ((rc = refpass_run(r, st)) != FFTSET_OK)
+                break;
     } else if (mode == FFTSET_BASELINE_PER_TRANSFORM) {
         // one launch per transform and step (job_ranges holds 3 per transform)
         for (const TileRange& r : b->job_ranges) {
@@ -828,8 +849,9 @@ void fftset_batch_destroy(struct fftset_batch* b)
         (void)hipEventDestroy(b->fork);
     if (b->mid)
         (void)hipEventDestroy(b->mid);
-    if (b->dev_mem || b->work_pool || b->pass_work)
+    if (b->dev_mem || b->work_pool || b->pass_work || !b->refpass.empty())
         (void)hipDeviceSynchronize();
+    for (RefPassClass& r : b->refpass) refpass_free(&r);
     if (b->dev_mem)
         (void)hipFree(b->dev_mem);
     if (b->work_pool)

@@ -98,6 +98,34 @@ hipError_t launch_ragged(const RaggedArgs& a, int threads, int lds_bytes, hipStr
 int pass_steps(int op, int np);
 hipError_t launch_pass(const RClass* d_cls, int step, int ntrans, int T, float2* work, hipStream_t st);
 
+// Per-size-launch baseline with the reference's pass structure
+// (kernels_refpass.hip): one launch per sweep over every transform of a class.
+enum RpKind : int { RPK_DIF = 0, RPK_DIT = 1, RPK_STOCK = 2, RPK_MULCONJ = 3, RPK_INTER = 4, RPK_DEINTER = 5 };
+// Where a sweep reads or writes: the work row, or the transform's input /
+// output (FREQOFFSETREAL: modulated / demodulated; COMPLEX: plain / conj).
+enum RpIo : int { RP_WORK = 0, RP_IN_FO = 1, RP_IN_CX = 2, RP_OUT_FO = 3, RP_OUT_CX = 4, RP_IN_OUT_CX = 5 };
+
+struct RpArgs {
+    int L;           // complex bins
+    int R, m, s;     // the pass (in-place view; Stockham: blocks per lane)
+    int V;           // sub-transforms (lanes) per transform
+    int conj_kern;   // product with conj(K) (signal adjoint)
+    long long ntrans;
+    const float2* tw;    // tw[(k-1) m + j] = W_{Rm}^{jk}; null when m == 1
+    const float2* tmod;  // FREQOFFSETREAL modulation W_{4L}^e
+    const float* const* in;  // per-transform input pointers (device array)
+    float* const* out;       // per-transform output pointers
+    float2* work;    // ntrans * L: in-place passes, Stockham source
+    float2* work2;   // ntrans * L: Stockham destination
+    const float2* kref;  // ntrans * L: kernel spectra in this structure's order
+    int slot[8];     // lane slot holding residue l (forward reorder)
+    int resid[8];    // residue held by lane slot q (inverse de-interleave)
+};
+
+hipError_t launch_refpass(int kind, int R, int io, const RpArgs& a, hipStream_t st);
+// kref-style gather at set-up: a.work[t][p] = ((float2*)a.in[t])[perm[p]].
+hipError_t launch_refpass_gather(const RpArgs& a, const int* perm, hipStream_t st);
+
 // ------------------------------------------------------- block convolution
 // Overlap-save FIR filtering of long real signals (see kernels_block.hip).
 struct BlockArgs {

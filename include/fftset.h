@@ -196,9 +196,15 @@ struct fftset_batch;
 
 #define FFTSET_BASELINE_PER_GROUP      1  /* one launch per (plan, operation) group and step */
 #define FFTSET_BASELINE_PER_TRANSFORM  2  /* one launch per transform and step               */
-#define FFTSET_BASELINE_PER_PASS       3  /* the reference's pass structure: one launch per radix
-                                             pass of each group, HBM round trip between passes
+#define FFTSET_BASELINE_PER_PASS       3  /* one launch per radix pass of this library's chain
+                                             for each group, HBM round trip between passes
                                              (four-step groups: one launch per step)           */
+#define FFTSET_BASELINE_REFERENCE_PASSES 4 /* the reference's pass structure for each group: its
+                                             modulating / splitting first pass, one sweep per
+                                             pass of its planner's chain at its vector width,
+                                             the separate product sweep of a convolution and
+                                             its reorder or demodulating last pass, one launch
+                                             per sweep (see fftset_baseline_chain)            */
 
 /* Creates an empty batch on the set; NULL on failure. */
 struct fftset_batch *fftset_batch_create(struct fftset *fc);
@@ -221,6 +227,17 @@ int fftset_batch_launch(struct fftset_batch *b, void *stream);
 /* Runs the same work as FFTSET_BASELINE_* separate launches (for
  * comparison). */
 int fftset_batch_launch_baseline(struct fftset_batch *b, int mode, void *stream);
+
+/* The pass structure FFTSET_BASELINE_REFERENCE_PASSES runs for
+ * (modulation, complex_bins), as the reference plans it
+ * (src/fftset_mod_freqoffsetreal.c:1012-1131, src/fftset_mod_cplx.c:320-347,
+ * src/fftset_vec.c:944-971, 1004-1062): the radices of the first pass's
+ * split into the vector width's sub-transforms ([4] at width 4, [4, 2] at
+ * width 8, none at width 1), a 0, then the planner's chain for length
+ * complex_bins / width. Returns the number of entries written (at most
+ * `max`), 0 for an unsupported size. Host only. */
+unsigned fftset_baseline_chain(const struct fftset_modulation *modulation, unsigned complex_bins,
+                               unsigned *radices, unsigned max);
 
 /* Algorithmic bytes of one launch: 16 L per forward / inverse / get_kernel,
  * 24 L per convolution. */

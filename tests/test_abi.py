@@ -89,3 +89,37 @@ def test_header_and_library_link_from_plain_c(tmp_path):
     r = subprocess.run(ge.c_dropin_command(str(out)), capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert out.exists()
+
+
+# The reference's schedules at the config sizes (its planner at its vector
+# widths, src/fftset_vec.c:1004-1062; widths chosen by
+# src/fftset_mod_freqoffsetreal.c:1012-1131 and src/fftset_mod_cplx.c:320-347):
+# modulation, L -> (first-pass split, inner chain).
+_REF_SCHEDULES = [
+    ("fo", 512, [4, 2], [4, 16]),
+    ("fo", 2048, [4, 2], [4, 4, 16]),
+    ("fo", 4096, [4, 2], [4, 4, 4, 8]),
+    ("cx", 3840, [4], [3, 4, 4, 4, 5]),
+    ("cx", 48, [4], [3, 4]), ("fo", 48, [4], [3, 4]),
+    ("cx", 120, [4], [2, 3, 5]), ("fo", 120, [], [2, 3, 4, 5]),
+    ("cx", 360, [4], [2, 3, 3, 5]), ("fo", 360, [], [2, 3, 3, 4, 5]),
+    ("cx", 768, [4], [3, 4, 16]), ("fo", 768, [4, 2], [4, 4, 6]),
+    ("cx", 1440, [4], [2, 3, 3, 4, 5]), ("fo", 1440, [4, 2], [3, 3, 4, 5]),
+    ("cx", 4608, [4], [3, 4, 4, 4, 6]), ("fo", 4608, [4, 2], [3, 3, 4, 16]),
+    ("cx", 15360, [4], [3, 4, 4, 4, 4, 5]), ("fo", 15360, [4, 2], [2, 3, 4, 4, 4, 5]),
+    ("cx", 61440, [4], [3, 4, 4, 4, 4, 4, 5]), ("fo", 61440, [4, 2], [2, 3, 4, 4, 4, 4, 5]),
+]
+
+
+@pytest.mark.parametrize("mod,L,split,inner", _REF_SCHEDULES)
+def test_baseline_chain_is_the_reference_schedule(mod, L, split, inner):
+    m = fftset_amd.FREQ_OFFSET_REAL if mod == "fo" else fftset_amd.COMPLEX
+    assert fftset_amd.baseline_chain(m, L) == (split, inner)
+
+
+def test_baseline_chain_matches_the_oracle_planner_at_width_one(oracle):
+    # at vector width 1 the reference's planner is the oracle's (which
+    # restates it for the scalar path): every width-1 structure agrees
+    for L in [2, 3, 5, 6, 9, 10, 15, 18, 27, 45, 90, 120, 135, 270, 360, 405, 1215, 3645]:
+        split, inner = fftset_amd.baseline_chain(fftset_amd.FREQ_OFFSET_REAL, L)
+        assert split == [] and inner == oracle.plan_chain(L), L

@@ -178,6 +178,49 @@ def test_per_pass_baseline_matches_the_fused_launch(fs, oracle):
     b.destroy()
 
 
+REF_SPEC = SPEC + [
+    # widths the ragged spec leaves out: width-1 COMPLEX (L % 4 != 0, and
+    # L = 4), width-1 FREQOFFSETREAL inverse, width-4 FREQOFFSETREAL
+    (CX, 90, fftset_amd.OP_FORWARD, 3), (CX, 90, fftset_amd.OP_CONV, 3), (CX, 45, fftset_amd.OP_INVERSE, 2),
+    (CX, 4, fftset_amd.OP_FORWARD, 3), (FO, 360, fftset_amd.OP_INVERSE, 2), (FO, 240, fftset_amd.OP_CONV, 3),
+    (FO, 80, fftset_amd.OP_FORWARD, 3), (FO, 240, fftset_amd.OP_CONV_GRAD_INPUT, 2),
+]
+
+
+def test_reference_pass_baseline_computes_the_same_transforms(fs):
+    # the reference's pass structure (first pass, its planner's chain at its
+    # vector width, product sweep, last pass; one launch per sweep) gives the
+    # fused launch's results to FP32 rounding and matches FP64; get_kernel
+    # jobs keep their spectra in the baseline's own order (not compared)
+    rng = np.random.default_rng(21)
+    jobs = _make_jobs(rng, REF_SPEC)
+    b = _build(fs, jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    fused = [jb["dy"].cpu().numpy().copy() for jb in jobs]
+    for rep in range(2):
+        for jb in jobs:
+            jb["dy"].fill_(float("nan"))
+        b.launch_baseline(fftset_amd.BASELINE_REFERENCE_PASSES)
+        torch.cuda.synchronize()
+        for jb, fy in zip(jobs, fused):
+            if jb["op"] == fftset_amd.OP_GET_KERNEL:
+                continue
+            tag = (jb["mod"], jb["L"], jb["op"], rep)
+            y = jb["dy"].cpu().numpy()
+            assert np.all(np.isfinite(y)), tag
+            assert fp64.rel_rms(y, fy) <= REF_TOL, tag
+            x, h, L = jb["x"], jb["h"], jb["L"]
+            if jb["op"] == fftset_amd.OP_FORWARD:
+                ref = fp64.fo_forward64(x) if jb["mod"] == FO else fp64.cx_forward64(x)
+            elif jb["op"] == fftset_amd.OP_CONV:
+                ref = fp64.fo_conv64(x, h) if jb["mod"] == FO else fp64.cx_conv64(x, h)
+            else:
+                continue
+            assert fp64.rel_rms(y, ref) <= 1e-6 * max(1.0, np.log2(L)), tag
+    b.destroy()
+
+
 def test_batch_four_step_classes_equal_drop_in_calls(fs):
     # four-step classes run the drop-in four-step kernels through the
     # class's pointer arrays: outputs equal single-transform drop-in calls bit
