@@ -61,6 +61,9 @@ def parse():
                     help="ragged: order of the tile groups inside the graph")
     ap.add_argument("--baselines", default="group,pass,reference,transform",
                     help="ragged: comma list of baselines to time (group, pass, reference, transform, none)")
+    ap.add_argument("--four-step-fused", action="store_true",
+                    help="run 61440-bin transforms with one workgroup per transform (intermediate in a per-workgroup "
+                         "scratch slot) instead of separate column and row launches")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
@@ -527,6 +530,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.four_step_fused:
+        fftset_amd.kernel_select(fftset_amd.SELECT_FOUR_STEP_FUSED)
     tune = args.tune_file if args.tune_file and os.path.exists(args.tune_file) else None
     fs = fftset_amd.FFTSet(tune_file=tune)
     rng = np.random.default_rng(0)

@@ -163,6 +163,7 @@ def _check(what):
 
 SELECT_GENERIC = 1
 SELECT_PERSISTENT = 2
+SELECT_FOUR_STEP_FUSED = 4
 
 
 def kernel_select(flags):

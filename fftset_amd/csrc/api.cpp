@@ -14,6 +14,7 @@
 #include <new>
 
 #include "context.h"
+#include "fs1.h"
 #include "planner.h"
 
 static const fftset_modulation g_mod_fo = {fsa::MOD_FO};
@@ -330,6 +331,30 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
         a.out_st = ost;
     };
     auto ok = [](hipError_t e) { return e == hipSuccess; };
+    if (op != OP_KERN && four_step_fused() && !persistent_mode() && !generic_forced() &&
+        fs1_matches(fs.colc, fs.rowc, fs.cols.T, fs.rows.T)) {
+        // one workgroup per transform; work holds its scratch slots (at most
+        // count of them, 2L floats each)
+        Fs1Args f{};
+        f.colc = fs.colc;
+        f.rowc = fs.rowc;
+        f.tmod = p->tmod;
+        f.tfs = fs.tfs;
+        f.tfs_fo = fs.tfs_fo;
+        f.trow = fs.trow_fo;
+        f.ntrans = count;
+        f.conj_kern = op == OP_GRAD_IN;
+        f.in = in;
+        f.in_st = in_st;
+        f.out = out;
+        f.out_st = out_st;
+        f.kern = kern;
+        f.kern_st = kern_st;
+        f.scratch = (float2*)work;
+        count_launch(true);
+        const int opk = op == OP_FWD ? FS1_FWD : op == OP_INV ? FS1_INV : FS1_CONV;
+        return ok(launch_fs1(p->mod, opk, f, fs1_grid(count), st)) ? FFTSET_OK : FFTSET_E_DEVICE;
+    }
     switch (op) {
     case OP_FWD:
         io(ca, in, in_st, work, work_st);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "context.h"
+#include "fs1.h"
 #include "planner.h"
 #include "refpass.h"
 
@@ -44,6 +45,7 @@ struct BatchClass {
     int col_threads, row_threads;  // four-step: column and row tiles
     std::vector<int> group_idx;    // this class's entries of group_ranges
     long long pass_off = 0;        // per-pass baseline: offset of its work tiles (float2)
+    float* work_base = nullptr;    // four-step: its first work row in the pool (host view)
 };
 
 // A contiguous run of tiles launched together with one workgroup size.
@@ -365,6 +367,8 @@ static int commit(fftset_batch* b, hipStream_t st)
         const size_t nj = k.jobs.size();
         const size_t o_in = o_ptr[4 * ci], o_out = o_ptr[4 * ci + 1];
         const size_t o_k = o_ptr[4 * ci + 2], o_w = o_ptr[4 * ci + 3];
+        if (k.four_step && k.op != OP_KERN)
+            b->classes[ci].work_base = wp;
         const void** hin = (const void**)(host.data() + o_in);
         void** hout = (void**)(host.data() + o_out);
         const void** hk = (const void**)(host.data() + o_k);
@@ -426,6 +430,13 @@ static int launch_list(const fftset_batch* b, const std::vector<TileRange>& rs, 
 // the drop-in four-step kernels, which read the class's per-transform pointer
 // arrays: the same kernels, tile shapes and data order as run_four_step.
 // j0 / n: a sub-range of the class's transforms (n < 0: all of them).
+// True when the class runs the one-workgroup-per-transform kernel.
+static bool fs1_class(const BatchClass& k)
+{
+    return k.four_step && k.op != OP_KERN && four_step_fused() && !persistent_mode() && !generic_forced() &&
+           fs1_matches(k.fs.colc, k.fs.rowc, k.fs.cols.T, k.fs.rows.T);
+}
+
 static int launch_fs_class(const BatchClass& k, int q, hipStream_t st, long long j0 = 0, long long n = -1)
 {
     RClass r = k.rc;
@@ -446,6 +457,28 @@ static int launch_fs_class(const BatchClass& k, int q, hipStream_t st, long long
     a.B = r.B;
     a.ntrans = n < 0 ? (long long)k.jobs.size() - j0 : n;
     a.conj_kern = r.op == OP_GRAD_IN;
+    if (fs1_class(k)) {
+        // one workgroup per transform, all steps in one launch (step 0);
+        // its scratch slots are the class's first work rows
+        if (q != 0)
+            return FFTSET_OK;
+        Fs1Args f{};
+        f.colc = fs.colc;
+        f.rowc = fs.rowc;
+        f.tmod = r.tmod;
+        f.tfs = fs.tfs;
+        f.tfs_fo = fs.tfs_fo;
+        f.trow = fs.trow_fo;
+        f.ntrans = a.ntrans;
+        f.conj_kern = a.conj_kern;
+        f.in_p = r.in;
+        f.out_p = r.out;
+        f.kern_p = conv ? r.kern : nullptr;
+        f.scratch = (float2*)(k.work_base + 2LL * r.L * j0);
+        count_launch(true);
+        const int opk = r.op == OP_FWD ? FS1_FWD : r.op == OP_INV ? FS1_INV : FS1_CONV;
+        return launch_fs1(r.mod, opk, f, fs1_grid(a.ntrans), st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
+    }
     const ShapeD& sh = q == 1 ? fs.rows : fs.cols;
     a.c = q == 1 ? fs.rowc : fs.colc;
     a.C = sh.G;
@@ -592,9 +625,11 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
     if (!fsc.empty()) {
         if (int rc = launch_fs_class(*fsc[0], 0, st))
             return rc;
-        if (hipEventRecord(b->mid, st) != hipSuccess)
-            return FFTSET_E_DEVICE;
-        after = b->mid;
+        if (!fs1_class(*fsc[0])) {  // one workgroup per transform: no column step to wait for
+            if (hipEventRecord(b->mid, st) != hipSuccess)
+                return FFTSET_E_DEVICE;
+            after = b->mid;
+        }
     }
     // the side branch after the other chains' (shared only beyond three classes)
     if (int rc = lds_from(std::min<int>(std::max<int>((int)fsc.size() - 1, 0), 2), after, false))

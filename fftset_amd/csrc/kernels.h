@@ -53,6 +53,7 @@ struct FsArgs {
 
 hipError_t launch_cols(int load, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);
 hipError_t launch_rows(int kind, int store, const FsArgs& a, const ShapeD& sh, hipStream_t st);
+
 // ------------------------------------------------------------ ragged batch
 // A ragged batch is a table of tiles over a table of classes (one class per
 // (plan, operation)); one workgroup runs one tile. Tiles are sorted by cost,
@@ -157,6 +158,7 @@ int persistent_waves();
 // of specialised (compile-time chain) and generic (runtime chain) kernels.
 bool persistent_mode();
 bool generic_forced();
+bool four_step_fused();
 void count_launch(bool specialised);
 
 // out[r*cols + c] = W_N^{((a0 + r) * c) mod N}, computed in FP64.

@@ -4,9 +4,11 @@
 // four-step chains, the block-convolution frame kernel) run their
 // specialised kernel unless FFTSET_SELECT_GENERIC is set; FFTSET_SELECT_
 // PERSISTENT switches the LDS and four-step launches to resident grids with
-// register prefetch. The environment (FFTSET_AMD_GENERIC, FFTSET_AMD_PERSIST)
-// gives the initial value, read once when the library loads; launches read
-// an atomic, never the environment.
+// register prefetch; FFTSET_SELECT_FOUR_STEP_FUSED runs 61440-bin transforms
+// with one workgroup per transform instead of column and row launches.
+// The environment (FFTSET_AMD_GENERIC, FFTSET_AMD_PERSIST) gives the initial
+// value, read once when the library loads; launches read an atomic, never
+// the environment.
 #include <atomic>
 #include <cstdlib>
 
@@ -34,6 +36,11 @@ bool persistent_mode() { return (g_select.load(std::memory_order_relaxed) & FFTS
 
 bool generic_forced() { return (g_select.load(std::memory_order_relaxed) & FFTSET_SELECT_GENERIC) != 0; }
 
+bool four_step_fused()
+{
+    return (g_select.load(std::memory_order_relaxed) & FFTSET_SELECT_FOUR_STEP_FUSED) != 0;
+}
+
 void count_launch(bool specialised)
 {
     (specialised ? g_specialised : g_generic).fetch_add(1, std::memory_order_relaxed);
@@ -45,7 +52,8 @@ extern "C" {
 
 unsigned fftset_kernel_select(unsigned flags)
 {
-    return fsa::g_select.exchange(flags & (FFTSET_SELECT_GENERIC | FFTSET_SELECT_PERSISTENT));
+    return fsa::g_select.exchange(flags &
+                                  (FFTSET_SELECT_GENERIC | FFTSET_SELECT_PERSISTENT | FFTSET_SELECT_FOUR_STEP_FUSED));
 }
 
 void fftset_kernel_counts(unsigned long long* specialised, unsigned long long* generic)

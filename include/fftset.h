@@ -302,11 +302,15 @@ void fftset_blockconv_destroy(struct fftset_blockconv *bc);
 /* Kernel selection, process-wide (for tests and comparisons). Chains with a
  * compile-time specialisation run it by default; FFTSET_SELECT_GENERIC runs
  * the runtime-chain kernels instead, FFTSET_SELECT_PERSISTENT resident-size
- * grids with register prefetch. Initial value from the environment
- * (FFTSET_AMD_GENERIC=1, FFTSET_AMD_PERSIST=1), read once at load. Returns
- * the previous flags. */
-#define FFTSET_SELECT_GENERIC    1u
-#define FFTSET_SELECT_PERSISTENT 2u
+ * grids with register prefetch. FFTSET_SELECT_FOUR_STEP_FUSED runs 61440-bin
+ * transforms with one workgroup per transform (intermediate in a
+ * per-workgroup scratch slot) instead of separate column and row launches
+ * (measured slower on MI355X; kept for comparison). Initial value from the
+ * environment (FFTSET_AMD_GENERIC=1, FFTSET_AMD_PERSIST=1), read once at
+ * load. Returns the previous flags. */
+#define FFTSET_SELECT_GENERIC          1u
+#define FFTSET_SELECT_PERSISTENT       2u
+#define FFTSET_SELECT_FOUR_STEP_FUSED  4u
 unsigned fftset_kernel_select(unsigned flags);
 /* Launches of specialised and of runtime-chain kernels issued by the
  * drop-in and uniform-batch calls since the previous call (which resets

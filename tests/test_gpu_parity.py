@@ -414,3 +414,47 @@ def test_specialised_kernels_match_generic(tuned_fs, mod, L, op):
     assert spec2 == 0 and gen2 >= 1, (spec2, gen2)
     assert np.all(np.isfinite(fixed))
     assert np.max(fp64.rel_rms(fixed, generic)) <= 1e-6
+
+
+@pytest.mark.parametrize("mod,op", [(CX, "fwd"), (FO, "fwd"), (CX, "inv"), (FO, "inv"), (FO, "conv"),
+                                    (CX, "conv"), (FO, "grad")])
+def test_one_workgroup_four_step_equals_split_launches(fs, mod, op):
+    # 61440 bins: the opt-in one-workgroup-per-transform kernel (intermediate
+    # in a per-workgroup scratch slot) against the default separate column and
+    # row launches (intermediate in the work buffer): the same passes,
+    # twiddles and order, so bit-identical; more transforms than workgroups,
+    # so slots are reused
+    import torch
+    L, n = 61440, 300
+    f = fs.create_fft(mod, L)
+    rng = np.random.default_rng(n + len(op))
+    x = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
+    h = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
+    k = torch.empty_like(h)
+    f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
+    code = {"fwd": fftset_amd.OP_FORWARD, "inv": fftset_amd.OP_INVERSE, "conv": fftset_amd.OP_CONV,
+            "grad": fftset_amd.OP_CONV_GRAD_INPUT}[op]
+
+    def run():
+        y = torch.full_like(x, float("nan"))
+        f.execute(code, n, y, x, kern=k if op in ("conv", "grad") else None)
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    split = run()
+    prev = fftset_amd.kernel_select(fftset_amd.SELECT_FOUR_STEP_FUSED)
+    try:
+        fused = run()
+    finally:
+        fftset_amd.kernel_select(prev)
+    assert np.all(np.isfinite(fused))
+    assert np.array_equal(fused, split)
+    xs = x.cpu().numpy()[[0, n - 1]]
+    if op == "fwd":
+        ref = fp64.fo_forward64(xs) if mod == FO else fp64.cx_forward64(xs)
+    elif op == "conv":
+        hs = h.cpu().numpy()[[0, n - 1]]
+        ref = fp64.fo_conv64(xs, hs) if mod == FO else fp64.cx_conv64(xs, hs)
+    else:
+        return
+    assert np.max(fp64.rel_rms(fused[[0, n - 1]], ref)) <= fp64_tol(L)
