@@ -165,6 +165,13 @@ int plan_lds_with(fftset_fft* p, Ctx* ctx, hipStream_t st, const std::vector<int
     int rc = attach_modulation(p, ctx, st);
     if (rc != FFTSET_OK)
         return rc;
+    p->tfo = nullptr;
+    if (p->mod == MOD_FO && p->chain.np >= 2) {
+        // W_{4L}^{u (1 + 4k)}: rows k < R0, columns u < L / R0
+        p->tfo = get_table(ctx, 4LL * L, 1, p->chain.p[0].R, p->chain.p[0].m, st, 4);
+        if (!p->tfo)
+            return FFTSET_E_NOMEM;
+    }
     p->engine = ENG_LDS;
     return FFTSET_OK;
 }
@@ -184,6 +191,7 @@ static int run_lds(const fftset_fft* p, int op, long long count, float* out, lon
     LdsArgs a;
     a.c = p->chain;
     a.tmod = p->tmod;
+    a.tfo = p->tfo;
     a.L = (int)p->lfft;
     a.ntrans = count;
     a.G = sh.G;

@@ -132,6 +132,7 @@ static int plan_class(fftset_batch* b, BatchClass& k, hipStream_t)
     r.op = k.op;
     r.L = (int)p->lfft;
     r.tmod = p->tmod;
+    r.tfo = p->tfo;
     k.four_step = p->engine == ENG_FOUR_STEP;
     if (!k.four_step) {
         if (p->shape.E != 16)

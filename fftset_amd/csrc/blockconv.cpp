@@ -107,6 +107,7 @@ int fftset_blockconv_run(struct fftset_blockconv* bc, float* out, size_t out_str
     BlockArgs a{};
     a.c = p->chain;
     a.tmod = p->tmod;
+    a.tfo = p->tfo;
     a.L = (int)p->lfft;
     a.n = (long long)nsamples;
     a.b = bc->b;

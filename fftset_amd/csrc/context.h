@@ -100,6 +100,9 @@ struct fftset_fft {
     fsa::ShapeD shape;  // LDS engine launch shape for large batches
     fsa::FourStep fs;   // ENG_FOUR_STEP
     const float2* tmod;  // FREQOFFSETREAL modulation W_{4L}^e, e < L (null for COMPLEX)
+    // FREQOFFSETREAL LDS plans with two or more passes: the first pass's
+    // folded modulation table W_{4L}^{u (1 + 4k)} (k < R0, u < L / R0)
+    const float2* tfo = nullptr;
 };
 
 namespace fsa {

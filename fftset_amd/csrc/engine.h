@@ -50,6 +50,22 @@ struct TileInter {
 // NB = butterflies a thread may own in one pass; u = lt + q*T for q < NB.
 // The host sizes T so that T*NB >= butterflies of every pass.
 
+// FREQOFFSETREAL modulation folded into the first pass (and demodulation
+// into the last DIT pass). In the first pass (radix R, m = n/R, s = 1) the
+// inputs of butterfly u are e = u + t m, and W_{4n}^e = W_{4n}^u W_{4R}^t:
+// the second factor is a compile-time root per input, the first is common
+// to the butterfly's DFT and merges with its output twiddles W_n^{uk} into
+// one table tfo[k m + u] = W_{4n}^{u (1 + 4k)}, k < R. The closing DIT pass
+// mirrors it: inputs times tfo[t m + u], outputs times W_{4R}^k, which is
+// the demodulation W_{4n}^f of output f = u + k m. A functor opts in with
+// kFoFold = true and provides raw(e) = x[e] - i x[n+e] (load) or
+// put_demod(f, v) (store) and tfo; its operator() keeps the full modulation
+// for every other pass that reaches it.
+template <class F, class = void>
+struct FoFold : std::false_type {};
+template <class F>
+struct FoFold<F, std::void_t<decltype(F::kFoFold)>> : std::bool_constant<F::kFoFold> {};
+
 // First pass (s = 1, m = n/R): reads x[j + q m] through `ld`, writes the
 // tile at j + k m. Shared by Stockham and in-place DIF.
 template <int R, int NB, class Tile, class Load>
@@ -57,6 +73,27 @@ __device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m,
                                            const Load& ld)
 {
     float2 v[NB][R];
+    if constexpr (FoFold<Load>::value) {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int u = lt + q * T;
+            if (u < m)
+                static_for<R>([&](auto Tt) {
+                    constexpr int t = decltype(Tt)::value;
+                    v[q][t] = rootmul<t, 4 * R>(ld.raw(u + t * m));
+                });
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int u = lt + q * T;
+            if (u < m) {
+                DFT<R>::run(v[q]);
+#pragma unroll
+                for (int k = 0; k < R; ++k) tl.at(u + k * m) = cmul(v[q][k], ld.tfo[k * m + u]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int u = lt + q * T;
@@ -232,6 +269,23 @@ template <int R, int NB, class Tile, class Store>
 __device__ __forceinline__ void dit_last(const Tile& tl, int lt, int T, int m, const float2* __restrict__ tw,
                                          const Store& st)
 {
+    if constexpr (FoFold<Store>::value) {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int u = lt + q * T;
+            if (u < m) {
+                float2 v[R];
+#pragma unroll
+                for (int t = 0; t < R; ++t) v[t] = cmul(tl.at(u + t * m), st.tfo[t * m + u]);
+                DFT<R>::run(v);
+                static_for<R>([&](auto Kk) {
+                    constexpr int k = decltype(Kk)::value;
+                    st.put_demod(u + k * m, rootmul<k, 4 * R>(v[k]));
+                });
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int u = lt + q * T;
@@ -332,6 +386,18 @@ struct LdFoMod {
     }
 };
 
+// LdFoMod with the modulation folded into the first pass (see FoFold):
+// tfo[k m + u] = W_{4L}^{u (1 + 4k)} for the plan's first pass.
+struct LdFoFold {
+    static constexpr bool kFoFold = true;
+    const float* __restrict__ x;
+    int L;
+    const float2* __restrict__ tmod;
+    const float2* __restrict__ tfo;
+    __device__ __forceinline__ float2 raw(int e) const { return make_float2(ldg(x + e), -ldg(x + L + e)); }
+    __device__ __forceinline__ float2 operator()(int e) const { return LdFoMod{x, L, tmod}(e); }
+};
+
 // FREQOFFSETREAL inverse input (src/fftset_mod_freqoffsetreal.c:985-998):
 // w[e] = conj X[2e] for e < ceil(L/2), else X[2(L-1-e)+1].
 struct LdFoBins {
@@ -380,6 +446,22 @@ struct StFoDemod {
         stg(y + f, v.x * w.x - v.y * w.y);
         stg(y + L + f, v.x * w.y + v.y * w.x);
     }
+};
+
+// StFoDemod with the demodulation folded into the closing DIT pass (see
+// FoFold): put_demod(f, v) receives v already times W_{4L}^f.
+struct StFoFold {
+    static constexpr bool kFoFold = true;
+    float* __restrict__ y;
+    int L;
+    const float2* __restrict__ tmod;
+    const float2* __restrict__ tfo;
+    __device__ __forceinline__ void put_demod(int f, float2 v) const
+    {
+        stg(y + f, v.x);
+        stg(y + L + f, v.y);
+    }
+    __device__ __forceinline__ void operator()(int f, float2 v) const { StFoDemod{y, L, tmod}(f, v); }
 };
 
 template <bool CONJ>

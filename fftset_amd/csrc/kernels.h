@@ -10,6 +10,7 @@ namespace fsa {
 struct LdsArgs {
     ChainD c;
     const float2* tmod;  // FREQOFFSETREAL modulation: tmod[e] = W_{4L}^e, e < L
+    const float2* tfo;   // FREQOFFSETREAL first / closing pass: tfo[k m + u] = W_{4L}^{u (1 + 4k)}
     int L;               // complex bins (== c.n)
     long long ntrans;  // transforms in the batch
     int G, T;          // transforms per workgroup, lanes per transform
@@ -69,6 +70,7 @@ struct RClass {
     ChainD c;   // LDS chain or column chain (length A)
     ChainD c2;  // row chain (length B)
     const float2* tmod;
+    const float2* tfo;  // FREQOFFSETREAL LDS classes: folded first / closing pass table
     const float2* tfs;
     const float* const* in;  // per-transform buffers of this class
     float* const* out;
@@ -132,6 +134,7 @@ hipError_t launch_refpass_gather(const RpArgs& a, const int* perm, hipStream_t s
 struct BlockArgs {
     ChainD c;
     const float2* tmod;
+    const float2* tfo;   // folded first / closing pass table of the plan
     int L;               // complex bins; frames are N = 2L samples
     long long ntrans;    // channels * nblocks frames
     int G, T;

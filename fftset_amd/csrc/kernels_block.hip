@@ -21,12 +21,18 @@ namespace fsa {
 // First pass: frame sample s reads x[s] for lo <= s < n, else 0; then the
 // FREQOFFSETREAL modulation of LdFoMod. These are cached loads: the k - 1
 // overlap samples of a frame are read again by the next block's frame.
+// FOLD: the modulation folded into the first pass (engine.h), where the
+// kernel's registers allow it.
+template <bool FOLD>
 struct LdFrame {
+    static constexpr bool kFoFold = FOLD;
     const float* __restrict__ x;  // channel base (sample 0)
     long long s0, lo, n;
     int L;
     const float2* __restrict__ tmod;
+    const float2* __restrict__ tfo;
     __device__ __forceinline__ float sample(long long s) const { return (s >= lo && s < n) ? x[s] : 0.f; }
+    __device__ __forceinline__ float2 raw(int e) const { return make_float2(sample(s0 + e), -sample(s0 + L + e)); }
     __device__ __forceinline__ float2 operator()(int e) const
     {
         const float re = sample(s0 + e), im = sample(s0 + L + e);
@@ -37,11 +43,20 @@ struct LdFrame {
 
 // Last pass: the demodulation of StFoDemod, keeping frame positions
 // p >= k - 1 that land inside the signal.
+// FOLD: the demodulation folded into the closing pass (engine.h).
+template <bool FOLD>
 struct StFrame {
+    static constexpr bool kFoFold = FOLD;
     float* __restrict__ y;  // channel base (sample 0)
     long long o0, n;        // output sample of frame position 0; signal length
     int L, skip;            // skip = k - 1
     const float2* __restrict__ tmod;
+    const float2* __restrict__ tfo;
+    __device__ __forceinline__ void put_demod(int f, float2 v) const
+    {
+        put(f, v.x);
+        put(L + f, v.y);
+    }
     __device__ __forceinline__ void put(int p, float v) const
     {
         const long long o = o0 + p;
@@ -74,8 +89,9 @@ __global__ void __launch_bounds__(1024) k_block(const BlockArgs a)
     const long long c = t0 / a.nblocks;
     const long long i = t0 - c * a.nblocks;
     const long long s0 = i * a.b - a.skip;
-    const LdFrame ld{a.in + c * a.in_st, s0, a.lo, a.n, a.L, a.tmod};
-    const StFrame st{a.out + c * a.out_st, s0, a.n, a.L, a.skip, a.tmod};
+    // folding both sides exceeds 128 VGPRs here; the first pass gains most
+    const LdFrame<true> ld{a.in + c * a.in_st, s0, a.lo, a.n, a.L, a.tmod, a.tfo};
+    const StFrame<false> st{a.out + c * a.out_st, s0, a.n, a.L, a.skip, a.tmod, a.tfo};
     const BotConv bot{(const float2*)a.kern, false, true};
     run_conv<E>(a.c, tl, lt, T, ld, st, bot);
 }
@@ -109,8 +125,9 @@ __global__ void __launch_bounds__(1024) k_block_fix(const BlockArgs a)
     const long long c = t0 / a.nblocks;
     const long long i = t0 - c * a.nblocks;
     const long long s0 = i * a.b - a.skip;
-    const LdFrame ld{a.in + c * a.in_st, s0, a.lo, a.n, n, a.tmod};
-    const StFrame st{a.out + c * a.out_st, s0, a.n, n, a.skip, a.tmod};
+    // folding both sides exceeds 128 VGPRs here; the closing pass fits
+    const LdFrame<false> ld{a.in + c * a.in_st, s0, a.lo, a.n, n, a.tmod, a.tfo};
+    const StFrame<true> st{a.out + c * a.out_st, s0, a.n, n, a.skip, a.tmod, a.tfo};
     const BotConv bot{(const float2*)a.kern, false, true};
     const float2* tw0 = a.c.p[0].tw;
     const float2* tw1 = a.c.p[1].tw;

@@ -34,7 +34,7 @@ struct LdsIo {
                 return LdCx<true>{(const float2*)in};
         } else {
             if constexpr (MOD == MOD_FO)
-                return LdFoMod{in, a.L, a.tmod};
+                return LdFoFold{in, a.L, a.tmod, a.tfo};
             else
                 return LdCx<false>{(const float2*)in};
         }
@@ -47,9 +47,14 @@ struct LdsIo {
                 return StFoBins{(float2*)out, a.L};
             else
                 return StCx<false>{(float2*)out};
-        } else {
+        } else if constexpr (OP == OP_INV) {
             if constexpr (MOD == MOD_FO)
                 return StFoDemod{out, a.L, a.tmod};
+            else
+                return StCx<true>{(float2*)out};
+        } else {
+            if constexpr (MOD == MOD_FO)
+                return StFoFold{out, a.L, a.tmod, a.tfo};
             else
                 return StCx<true>{(float2*)out};
         }

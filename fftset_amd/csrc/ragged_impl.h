@@ -17,7 +17,10 @@ namespace fsa {
 constexpr int kRaggedE = 16;
 constexpr int kIdleLane = 1 << 28;
 
-template <int MOD, int OP>
+// FOLD_ST: the FREQOFFSETREAL demodulation folded into the convolution's
+// closing pass too (the 1024-lane kernel keeps it in the store: folding both
+// sides there exceeds 128 VGPRs).
+template <int MOD, int OP, bool FOLD_ST>
 __device__ __forceinline__ void ragged_lds(const RClass& k, const RTile& t, float2* smem)
 {
     const int T = k.T;
@@ -32,7 +35,7 @@ __device__ __forceinline__ void ragged_lds(const RClass& k, const RTile& t, floa
     const int L = k.L;
     if constexpr (OP == OP_FWD) {
         if constexpr (MOD == MOD_FO)
-            run_stockham<kRaggedE>(k.c, tl, lt, T, LdFoMod{in, L, k.tmod}, StFoBins{(float2*)out, L});
+            run_stockham<kRaggedE>(k.c, tl, lt, T, LdFoFold{in, L, k.tmod, k.tfo}, StFoBins{(float2*)out, L});
         else
             run_stockham<kRaggedE>(k.c, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<false>{(float2*)out});
     } else if constexpr (OP == OP_INV) {
@@ -42,13 +45,16 @@ __device__ __forceinline__ void ragged_lds(const RClass& k, const RTile& t, floa
             run_stockham<kRaggedE>(k.c, tl, lt, T, LdCx<true>{(const float2*)in}, StCx<true>{(float2*)out});
     } else if constexpr (OP == OP_KERN) {
         if constexpr (MOD == MOD_FO)
-            run_kern<kRaggedE>(k.c, tl, lt, T, LdFoMod{in, L, k.tmod}, (float2*)out);
+            run_kern<kRaggedE>(k.c, tl, lt, T, LdFoFold{in, L, k.tmod, k.tfo}, (float2*)out);
         else
             run_kern<kRaggedE>(k.c, tl, lt, T, LdCx<false>{(const float2*)in}, (float2*)out);
     } else {
         const BotConv bot{(const float2*)k.kern[j], OP == OP_GRAD_IN};
-        if constexpr (MOD == MOD_FO)
-            run_conv<kRaggedE>(k.c, tl, lt, T, LdFoMod{in, L, k.tmod}, StFoDemod{out, L, k.tmod}, bot);
+        if constexpr (MOD == MOD_FO && FOLD_ST)
+            run_conv<kRaggedE>(k.c, tl, lt, T, LdFoFold{in, L, k.tmod, k.tfo}, StFoFold{out, L, k.tmod, k.tfo},
+                               bot);
+        else if constexpr (MOD == MOD_FO)
+            run_conv<kRaggedE>(k.c, tl, lt, T, LdFoFold{in, L, k.tmod, k.tfo}, StFoDemod{out, L, k.tmod}, bot);
         else
             run_conv<kRaggedE>(k.c, tl, lt, T, LdCx<false>{(const float2*)in}, StCx<true>{(float2*)out}, bot);
     }
@@ -145,11 +151,11 @@ __device__ __forceinline__ void ragged_row(const RClass& k, const RTile& t, floa
 
 // LDS tiles and four-step tiles are compiled into separate kernels (each
 // launch range holds one kind), which keeps each kernel's code small.
-template <int MOD, int OP, bool FS>
+template <int MOD, int OP, bool FS, bool FOLD_ST>
 __device__ __forceinline__ void ragged_dispatch_kind(const RClass& k, const RTile& t, float2* smem)
 {
     if constexpr (!FS) {
-        ragged_lds<MOD, OP>(k, t, smem);
+        ragged_lds<MOD, OP, FOLD_ST>(k, t, smem);
     } else {
         switch (t.kind) {
         case TK_COL: ragged_col<MOD, OP, false>(k, t, smem); break;
@@ -163,23 +169,23 @@ __device__ __forceinline__ void ragged_dispatch_kind(const RClass& k, const RTil
     }
 }
 
-template <bool FS>
+template <bool FS, bool FOLD_ST>
 __device__ __forceinline__ void ragged_tile(const RaggedArgs& a, int ti, float2* smem)
 {
     const RTile t = a.tiles[ti];
     const RClass& k = a.classes[t.cls];
     const int sel = k.mod * OP_COUNT + k.op;
     switch (sel) {
-    case MOD_FO * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_FO, OP_FWD, FS>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_FO, OP_INV, FS>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_FO, OP_KERN, FS>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_FO, OP_CONV, FS>(k, t, smem); break;
-    case MOD_FO * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_FO, OP_GRAD_IN, FS>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_CX, OP_FWD, FS>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_CX, OP_INV, FS>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_CX, OP_KERN, FS>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_CX, OP_CONV, FS>(k, t, smem); break;
-    case MOD_CX * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_CX, OP_GRAD_IN, FS>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_FO, OP_FWD, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_FO, OP_INV, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_FO, OP_KERN, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_FO, OP_CONV, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_FO * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_FO, OP_GRAD_IN, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_FWD: ragged_dispatch_kind<MOD_CX, OP_FWD, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_INV: ragged_dispatch_kind<MOD_CX, OP_INV, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_KERN: ragged_dispatch_kind<MOD_CX, OP_KERN, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_CONV: ragged_dispatch_kind<MOD_CX, OP_CONV, FS, FOLD_ST>(k, t, smem); break;
+    case MOD_CX * OP_COUNT + OP_GRAD_IN: ragged_dispatch_kind<MOD_CX, OP_GRAD_IN, FS, FOLD_ST>(k, t, smem); break;
     default: break;
     }
 }
@@ -203,7 +209,7 @@ __global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
             if (ti >= a.ntiles)
                 return;  // the same value in every lane: the whole workgroup leaves
         }
-        ragged_tile<FS>(a, ti, smem);
+        ragged_tile<FS, (THREADS < 1024)>(a, ti, smem);
         if (!a.next)
             return;
         __syncthreads();  // LDS and s_tile are reused by the next tile
