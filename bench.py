@@ -172,19 +172,25 @@ def workload_c5(torch, fs, rng_np, args):
     h = torch.randn(B, 2 * L, device="cuda", generator=g) * torch.exp(-t / 256.0)
     w = torch.randn(B, 2 * L, device="cuda", generator=g)
     K = torch.empty_like(h)
-    y, gx, gh, work = (torch.empty_like(x) for _ in range(4))
+    y, gx, gh, X = (torch.empty_like(x) for _ in range(4))
+    # one training step of fftset_amd.autograd.fft_conv: the kernel's
+    # spectrum, the forward convolution saving the signal's spectrum X, then
+    # the two adjoints (dL/dh is the signal adjoint with X in place of K)
     kernels = {
         "get_kernel": lambda: f.execute(fftset_amd.OP_GET_KERNEL, B, K, h),
-        "conv": lambda: f.execute(fftset_amd.OP_CONV, B, y, x, kern=K),
+        "conv_save_spectrum": lambda: f.execute(fftset_amd.OP_CONV_SAVE_SPECTRUM, B, y, x, kern=K, work=X),
         "grad_input": lambda: f.execute(fftset_amd.OP_CONV_GRAD_INPUT, B, gx, w, kern=K),
-        "grad_kernel": lambda: f.execute(fftset_amd.OP_CONV_GRAD_KERNEL, B, gh, x, kern=w, work=work),
+        "grad_kernel": lambda: f.execute(fftset_amd.OP_CONV_GRAD_INPUT, B, gh, w, kern=X),
     }
-    bytes_per = {"get_kernel": 16 * L * B, "conv": 24 * L * B, "grad_input": 24 * L * B,
+    # conv_save_spectrum: signal, kernel spectrum and output (24 L) plus the
+    # signal spectrum written (8 L)
+    bytes_per = {"get_kernel": 16 * L * B, "conv_save_spectrum": 32 * L * B, "grad_input": 24 * L * B,
                  "grad_kernel": 24 * L * B}
-    cfg = {"workload": "differentiable FREQOFFSETREAL convolution (forward + both adjoints)",
+    cfg = {"workload": "differentiable FREQOFFSETREAL convolution (forward saving the signal spectrum + both "
+                       "adjoints)",
            "complex_bins": L, "channels": B, "radices": f.radices(), "transforms_per_step": 4 * B,
-           # get_kernel + conv + grad_input + grad_kernel (a forward and a grad_input)
-           "nominal_flop_per_step": B * (2 * nominal_flop(L, "fwd") + 3 * nominal_flop(L, "conv"))}
+           # get_kernel (a forward) + conv + grad_input + grad_kernel (two convolution-shaped passes)
+           "nominal_flop_per_step": B * (nominal_flop(L, "fwd") + 3 * nominal_flop(L, "conv"))}
 
     def check():
         from oracle import fp64

@@ -33,6 +33,7 @@ OP_GET_KERNEL = 2
 OP_CONV = 3
 OP_CONV_GRAD_INPUT = 4
 OP_CONV_GRAD_KERNEL = 5
+OP_CONV_SAVE_SPECTRUM = 6
 
 _lib = None
 

@@ -6,8 +6,11 @@ through the C ABI, and its backward calls the library's adjoint kernels:
 
 * dLoss/dx = ``fftset_fft_conv_grad_input``(g, K): the product with the
   conjugate kernel spectrum (negacyclic correlation with h);
-* dLoss/dh = ``fftset_fft_conv_grad_kernel``(g, x): the spectrum product
-  W conj(X) followed by the inverse (negacyclic correlation of g with x).
+* dLoss/dh = the same adjoint with the signal's spectrum X in place of K
+  (the spectrum product G conj(X) followed by the inverse: negacyclic
+  correlation of g with x). When h needs a gradient the forward pass runs
+  FFTSET_OP_CONV_SAVE_SPECTRUM, which writes X beside y, so the backward is
+  one pass per gradient.
 
 Everything runs stream-ordered on torch's current stream; there is no
 PyTorch fallback -- the library must be built.
@@ -38,15 +41,22 @@ class FFTConv(torch.autograd.Function):
         K = torch.empty_like(h)
         fft.execute(fftset_amd.OP_GET_KERNEL, nh, K, h)
         y = torch.empty_like(x)
-        fft.execute(fftset_amd.OP_CONV, n, y, x, kern=K, kern_stride=0 if nh == 1 else 2 * L)
+        kst = 0 if nh == 1 else 2 * L
+        X = None
+        if ctx.needs_input_grad[1]:
+            X = torch.empty_like(x)  # the signal's spectrum, for dLoss/dh
+            fft.execute(fftset_amd.OP_CONV_SAVE_SPECTRUM, n, y, x, kern=K, kern_stride=kst, work=X)
+        else:
+            fft.execute(fftset_amd.OP_CONV, n, y, x, kern=K, kern_stride=kst)
         ctx.fft = fft
         ctx.shared_kernel = nh == 1
-        ctx.save_for_backward(x, K)
+        ctx.kernel_shape = h.shape
+        ctx.save_for_backward(X, K)
         return y
 
     @staticmethod
     def backward(ctx, gy):
-        x, K = ctx.saved_tensors
+        X, K = ctx.saved_tensors
         fft = ctx.fft
         L = fft.complex_bins
         gy, n = _rows(gy, L)
@@ -57,10 +67,9 @@ class FFTConv(torch.autograd.Function):
                         kern_stride=0 if ctx.shared_kernel else 2 * L)
         if ctx.needs_input_grad[1]:
             gh = torch.empty_like(gy)
-            work = torch.empty_like(gy)
-            fft.execute(fftset_amd.OP_CONV_GRAD_KERNEL, n, gh, x, kern=gy, work=work)
+            fft.execute(fftset_amd.OP_CONV_GRAD_INPUT, n, gh, gy, kern=X)
             if ctx.shared_kernel:
-                gh = gh.reshape(-1, 2 * L).sum(dim=0, keepdim=True).reshape(K.shape)
+                gh = gh.reshape(-1, 2 * L).sum(dim=0, keepdim=True).reshape(ctx.kernel_shape)
         return gx, gh, None
 
 

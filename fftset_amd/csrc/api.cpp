@@ -183,12 +183,13 @@ static int plan_lds(fftset_fft* p, Ctx* ctx, hipStream_t st)
 }
 
 static int run_lds(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
-                   long long in_st, const float* kern, long long kern_st, hipStream_t st)
+                   long long in_st, const float* kern, long long kern_st, hipStream_t st, float* spec,
+                   long long spec_st)
 {
     ShapeD sh = p->shape;
     if (count < sh.G && !choose_shape(p->chain, sh.E, count, &sh))
         return FFTSET_E_UNSUPPORTED;
-    LdsArgs a;
+    LdsArgs a{};
     a.c = p->chain;
     a.tmod = p->tmod;
     a.tfo = p->tfo;
@@ -202,6 +203,8 @@ static int run_lds(const fftset_fft* p, int op, long long count, float* out, lon
     a.in_st = in_st;
     a.out_st = out_st;
     a.kern_st = kern_st;
+    a.spec = spec;
+    a.spec_st = spec_st;
     return launch_lds(p->mod, op, a, sh, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
 }
 
@@ -307,7 +310,8 @@ static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
 
 static int run_four_step(const fftset_fft* p, int op, long long count, float* out, long long out_st,
                          const float* in, long long in_st, const float* kern, long long kern_st, float* work,
-                         long long work_st, hipStream_t st)
+                         long long work_st, hipStream_t st, float* spec,
+                         long long spec_st)
 {
     const FourStep& fs = p->fs;
     const bool fo = p->mod == MOD_FO;
@@ -339,7 +343,7 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
         a.out_st = ost;
     };
     auto ok = [](hipError_t e) { return e == hipSuccess; };
-    if (op != OP_KERN && four_step_fused() && !persistent_mode() && !generic_forced() &&
+    if (op != OP_KERN && !spec && four_step_fused() && !persistent_mode() && !generic_forced() &&
         fs1_matches(fs.colc, fs.rowc, fs.cols.T, fs.rows.T)) {
         // one workgroup per transform; work holds its scratch slots (at most
         // count of them, 2L floats each)
@@ -390,6 +394,8 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
         io(ra, work, work_st, work, work_st);
         ra.kern = kern;
         ra.kern_st = kern_st;
+        ra.spec = spec;  // the signal's spectrum, written by the row step's bottom
+        ra.spec_st = spec_st;
         FsArgs cb = ca;
         io(cb, work, work_st, out, out_st);
         if (!ok(launch_cols(fo ? CL_FOMODC : CL_CX, CS_WORKTW, ca, fs.cols, st)))
@@ -404,10 +410,10 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
 
 static int run_op(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
                   long long in_st, const float* kern, long long kern_st, float* work, long long work_st,
-                  hipStream_t st)
+                  hipStream_t st, float* spec = nullptr, long long spec_st = 0)
 {
     if (p->engine == ENG_LDS)
-        return run_lds(p, op, count, out, out_st, in, in_st, kern, kern_st, st);
+        return run_lds(p, op, count, out, out_st, in, in_st, kern, kern_st, st, spec, spec_st);
     const long long len2 = 2LL * p->lfft;
     std::unique_lock<std::mutex> lease;
     if (op != OP_KERN && !work) {
@@ -416,14 +422,14 @@ static int run_op(const fftset_fft* p, int op, long long count, float* out, long
             return rc;
         work_st = len2;
     }
-    return run_four_step(p, op, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st);
+    return run_four_step(p, op, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st, spec, spec_st);
 }
 
 int execute(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
             long long in_st, const float* kern, long long kern_st, float* work, long long work_st,
             hipStream_t st, float* work2)
 {
-    if (!p || count < 0 || op < 0 || op > FFTSET_OP_CONV_GRAD_KERNEL)
+    if (!p || count < 0 || op < 0 || op > FFTSET_OP_CONV_SAVE_SPECTRUM)
         return FFTSET_E_INVALID;
     if (count == 0)
         return FFTSET_OK;
@@ -432,10 +438,11 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
         return FFTSET_E_INVALID;
     if ((count > 1 && (out_st < len2 || in_st < len2)) || out_st < 0 || in_st < 0)
         return FFTSET_E_INVALID;
-    const bool needs_kern = op == FFTSET_OP_CONV || op == FFTSET_OP_CONV_GRAD_INPUT || op == FFTSET_OP_CONV_GRAD_KERNEL;
+    const bool needs_kern = op == FFTSET_OP_CONV || op == FFTSET_OP_CONV_GRAD_INPUT ||
+                            op == FFTSET_OP_CONV_GRAD_KERNEL || op == FFTSET_OP_CONV_SAVE_SPECTRUM;
     if (needs_kern && (!kern || (kern_st != 0 && kern_st < len2 && count > 1)))
         return FFTSET_E_INVALID;
-    if (((uintptr_t)out | (uintptr_t)in | (uintptr_t)kern) & 7)
+    if (((uintptr_t)out | (uintptr_t)in | (uintptr_t)kern | (uintptr_t)work) & 7)
         return FFTSET_E_INVALID;
 
     if (work && count > 1 && work_st < len2)
@@ -449,6 +456,13 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
         return run_op(p, OP_CONV, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st);
     case FFTSET_OP_CONV_GRAD_INPUT:
         return run_op(p, OP_GRAD_IN, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st);
+    case FFTSET_OP_CONV_SAVE_SPECTRUM:
+        // work receives the signal's spectrum; a four-step intermediate
+        // comes from the library's scratch
+        if (!work)
+            return FFTSET_E_INVALID;
+        return run_op(p, OP_CONV, count, out, out_st, in, in_st, kern, kern_st, nullptr, 0, st, work,
+                      count > 1 ? work_st : len2);
     case FFTSET_OP_CONV_GRAD_KERNEL: {
         // spectrum of the signal (in) into a scratch region w, then the
         // signal-adjoint pass of the output gradient (kern) against it.

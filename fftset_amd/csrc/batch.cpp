@@ -12,6 +12,11 @@
 //     one launch of the ragged kernel per step over all four-step classes).
 // The same kernels launched once per (class, step), or once per transform,
 // give the per-size-launch baselines.
+//
+// A kernel-gradient job (FFTSET_OP_CONV_GRAD_KERNEL) is expanded at commit
+// into a get_kernel of its signal into a spectrum row the batch owns
+// (stage 0) and a signal-adjoint pass of its output gradient against that
+// row (stage 1); stage 1 runs once all of stage 0 has finished.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,11 +37,20 @@ struct BatchJob {
     float* out;
     const float* in;
     const float* kern;
+    int stage = 0;          // 1: runs after every stage-0 job
+    bool internal = false;  // part of an expanded kernel-gradient job
+    bool expanded = false;  // a kernel-gradient job, run by its two internal jobs
 };
+
+// A kernel-gradient job as added (expanded at commit).
+constexpr int kOpGradKernelJob = 100;
 
 struct BatchClass {
     const fftset_fft* p;
     int op;
+    int stage = 0;
+    bool internal = false;
+    int spec_src = -1;      // stage-1 kernel-gradient class: the class writing its spectra
     std::vector<int> jobs;  // indices into the batch's job list
     RClass rc;              // device descriptor (pointers patched at upload)
     bool four_step;
@@ -77,6 +91,8 @@ struct fftset_batch {
     RTile* d_job_tiles = nullptr;    // per transform, for the per-transform baseline
     float* work_pool = nullptr;
     std::vector<TileRange> lds_r;    // LDS-resident tiles, one range per workgroup size
+    std::vector<TileRange> lds_r1;   // stage-1 LDS-resident tiles
+    float* spec_pool = nullptr;      // spectra of expanded kernel-gradient jobs
     std::vector<TileRange> fs_r[3];  // four-step steps (columns, rows, closing columns)
     int sched = 0;                   // FFTSET_BATCH_SCHED_*
     bool persistent = false;         // FFTSET_BATCH_PERSISTENT
@@ -200,12 +216,42 @@ struct CostTile {
 
 static int commit(fftset_batch* b, hipStream_t st)
 {
-    // classes in first-seen order of (plan, op)
+    // expand kernel-gradient jobs: all their get_kernel jobs, then all their
+    // adjoint passes, appended after the jobs as added
+    size_t spec_floats = 0;
+    for (const BatchJob& jb : b->jobs)
+        if (jb.op == kOpGradKernelJob)
+            spec_floats += 2ull * jb.p->lfft;
+    if (spec_floats) {
+        if (hipMalloc((void**)&b->spec_pool, spec_floats * sizeof(float)) != hipSuccess)
+            return FFTSET_E_NOMEM;
+        const size_t n0 = b->jobs.size();
+        std::vector<BatchJob> second;
+        float* sp = b->spec_pool;
+        for (size_t j = 0; j < n0; ++j) {
+            BatchJob& jb = b->jobs[j];
+            if (jb.op != kOpGradKernelJob)
+                continue;
+            jb.expanded = true;
+            // execute()'s convention: in = the signal, kern = the output gradient
+            BatchJob a{jb.p, OP_KERN, sp, jb.in, nullptr, 0, true, false};
+            BatchJob c{jb.p, OP_GRAD_IN, jb.out, jb.kern, sp, 1, true, false};
+            b->jobs.push_back(a);
+            second.push_back(c);
+            sp += 2ull * jb.p->lfft;
+        }
+        b->jobs.insert(b->jobs.end(), second.begin(), second.end());
+    }
+    // classes in first-seen order of (plan, op, stage, internal)
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const BatchJob& jb = b->jobs[j];
+        if (jb.expanded) {
+            b->alg_bytes += 24LL * jb.p->lfft;  // signal, output gradient, kernel gradient
+            continue;
+        }
         BatchClass* k = nullptr;
         for (auto& c : b->classes)
-            if (c.p == jb.p && c.op == jb.op) {
+            if (c.p == jb.p && c.op == jb.op && c.stage == jb.stage && c.internal == jb.internal) {
                 k = &c;
                 break;
             }
@@ -214,9 +260,17 @@ static int commit(fftset_batch* b, hipStream_t st)
             k = &b->classes.back();
             k->p = jb.p;
             k->op = jb.op;
+            k->stage = jb.stage;
+            k->internal = jb.internal;
         }
         k->jobs.push_back((int)j);
     }
+    // a stage-1 class reads the spectra its stage-0 twin writes, in job order
+    for (auto& k : b->classes)
+        if (k.stage == 1)
+            for (size_t c = 0; c < b->classes.size(); ++c)
+                if (b->classes[c].internal && b->classes[c].stage == 0 && b->classes[c].p == k.p)
+                    k.spec_src = (int)c;
     size_t work_floats = 0;
     for (auto& k : b->classes) {
         int rc = plan_class(b, k, st);
@@ -226,11 +280,12 @@ static int commit(fftset_batch* b, hipStream_t st)
             return FFTSET_E_UNSUPPORTED;
         if (k.four_step && k.op != OP_KERN)
             work_floats += 2ull * k.p->lfft * k.jobs.size();
-        b->alg_bytes += op_bytes(k.op, k.p->lfft) * (long long)k.jobs.size();
+        if (!k.internal)
+            b->alg_bytes += op_bytes(k.op, k.p->lfft) * (long long)k.jobs.size();
     }
 
     // tiles: fused (by phase, largest first), per group, per transform
-    std::vector<CostTile> ph[3];
+    std::vector<CostTile> ph[3], ph1;  // stage 0 by step; stage-1 LDS tiles
     std::vector<RTile> group, per_job;
     // per transform: one tile list per phase (each phase is its own launch)
     std::vector<std::vector<RTile>> job_tiles(3 * b->jobs.size());
@@ -245,7 +300,7 @@ static int commit(fftset_batch* b, hipStream_t st)
         if (!k.four_step) {
             for (int j = 0; j < nj; j += r.G) {
                 RTile t{ci, TK_LDS, j, std::min(r.G, nj - j)};
-                ph[0].push_back({t, per * t.count});
+                (k.stage ? ph1 : ph[0]).push_back({t, per * t.count});
                 g3[0].push_back(t);
             }
             for (int j = 0; j < nj; ++j) job_tiles[3 * k.jobs[j]].push_back(RTile{ci, TK_LDS, j, 1});
@@ -256,7 +311,8 @@ static int commit(fftset_batch* b, hipStream_t st)
             for (int j = 0; j < nj; ++j) {
                 for (int q = 0; q < r.colGroups; ++q) {
                     RTile t{ci, TK_COL, j, q};
-                    ph[0].push_back({t, col_cost});
+                    if (!k.stage)  // stage-1 four-step classes run their own launches
+                        ph[0].push_back({t, col_cost});
                     g3[0].push_back(t);
                     job_tiles[3 * k.jobs[j]].push_back(t);
                 }
@@ -264,7 +320,8 @@ static int commit(fftset_batch* b, hipStream_t st)
             for (int j = 0; j < nj; ++j) {
                 for (int q = 0; q < r.rowGroups; ++q) {
                     RTile t{ci, TK_ROW, j, q};
-                    ph[1].push_back({t, row_cost});
+                    if (!k.stage)
+                        ph[1].push_back({t, row_cost});
                     g3[1].push_back(t);
                     job_tiles[3 * k.jobs[j] + 1].push_back(t);
                 }
@@ -273,7 +330,8 @@ static int commit(fftset_batch* b, hipStream_t st)
                 for (int j = 0; j < nj; ++j) {
                     for (int q = 0; q < r.colGroups; ++q) {
                         RTile t{ci, TK_COL2, j, q};
-                        ph[2].push_back({t, col_cost});
+                        if (!k.stage)
+                            ph[2].push_back({t, col_cost});
                         g3[2].push_back(t);
                         job_tiles[3 * k.jobs[j] + 2].push_back(t);
                     }
@@ -313,7 +371,9 @@ static int commit(fftset_batch* b, hipStream_t st)
     for (int q = 0; q < 3; ++q)
         std::stable_sort(ph[q].begin(), ph[q].end(),
                          [](const CostTile& x, const CostTile& y) { return x.cost > y.cost; });
+    std::stable_sort(ph1.begin(), ph1.end(), [](const CostTile& x, const CostTile& y) { return x.cost > y.cost; });
     add_ranges(ph[0], true, b->lds_r);
+    add_ranges(ph1, true, b->lds_r1);
     for (int q = 0; q < 3; ++q) add_ranges(ph[q], false, b->fs_r[q]);
     for (size_t v = 0; v < job_tiles.size(); ++v) {
         const auto& tl = job_tiles[v];
@@ -334,6 +394,7 @@ static int commit(fftset_batch* b, hipStream_t st)
     // tile counters of the fused ranges (persistent launches)
     int ncounters = 0;
     for (TileRange& r : b->lds_r) r.counter = ncounters++;
+    for (TileRange& r : b->lds_r1) r.counter = ncounters++;
     for (int q = 0; q < 3; ++q)
         for (TileRange& r : b->fs_r[q]) r.counter = ncounters++;
 
@@ -523,7 +584,7 @@ static int launch_fs_step(const fftset_batch* b, int q, hipStream_t st)
     if (b->fs_tiles)
         return launch_list(b, b->fs_r[q], st);
     for (const BatchClass& k : b->classes) {
-        if (!k.four_step)
+        if (!k.four_step || k.stage)
             continue;
         if (int rc = launch_fs_class(k, q, st))
             return rc;
@@ -536,7 +597,7 @@ static std::vector<const BatchClass*> four_step_classes(const fftset_batch* b)
 {
     std::vector<const BatchClass*> fsc;
     for (const BatchClass& k : b->classes)
-        if (k.four_step)
+        if (k.four_step && !k.stage)
             fsc.push_back(&k);
     std::stable_sort(fsc.begin(), fsc.end(), [](const BatchClass* x, const BatchClass* y) {
         return op_bytes(x->op, x->rc.L) * (long long)x->jobs.size() >
@@ -558,7 +619,7 @@ static std::vector<const BatchClass*> four_step_classes(const fftset_batch* b)
 //    then the rows, then the closing columns;
 //  SERIAL: every group one after the other, largest workgroups first.
 // Without side streams (direct launches) every schedule runs serially.
-static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
+static int launch_stage0(const fftset_batch* b, hipStream_t st, bool branches)
 {
     const int sched = branches ? b->sched : FFTSET_BATCH_SCHED_SERIAL;
     if (sched == FFTSET_BATCH_SCHED_SERIAL) {
@@ -642,6 +703,24 @@ static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
     return join_all();
 }
 
+// Stage 1 (the adjoint passes of kernel-gradient jobs) after all of stage
+// 0: its LDS tiles, then its four-step classes, on the launch stream.
+static int launch_fused(const fftset_batch* b, hipStream_t st, bool branches)
+{
+    if (int rc = launch_stage0(b, st, branches))
+        return rc;
+    if (int rc = launch_list(b, b->lds_r1, st))
+        return rc;
+    for (const BatchClass& k : b->classes) {
+        if (!k.four_step || !k.stage)
+            continue;
+        for (int q = 0; q < 3; ++q)
+            if (int rc = launch_fs_class(k, q, st))
+                return rc;
+    }
+    return FFTSET_OK;
+}
+
 }  // namespace fsa
 
 extern "C" {
@@ -668,12 +747,12 @@ int fftset_batch_add(struct fftset_batch* b, const struct fftset_fft* fft, int o
         set_error(FFTSET_E_INVALID);
         return FFTSET_E_INVALID;
     }
-    const int iop = map_op(op);
+    const int iop = op == FFTSET_OP_CONV_GRAD_KERNEL ? kOpGradKernelJob : map_op(op);
     if (iop < 0) {
         set_error(FFTSET_E_UNSUPPORTED);
         return FFTSET_E_UNSUPPORTED;
     }
-    const bool needs_kern = iop == OP_CONV || iop == OP_GRAD_IN;
+    const bool needs_kern = iop == OP_CONV || iop == OP_GRAD_IN || iop == kOpGradKernelJob;
     if (!out || !in || (needs_kern && !kern) || (((uintptr_t)out | (uintptr_t)in | (uintptr_t)kern) & 7) ||
         (iop == OP_KERN && (const float*)out == in)) {
         set_error(FFTSET_E_INVALID);
@@ -807,7 +886,8 @@ int fftset_batch_launch_baseline(struct fftset_batch* b, int mode, void* stream)
         if (b->refpass.empty()) {
             for (const BatchClass& k : b->classes) {
                 RefPassClass r;
-                rc = refpass_setup(b->ctx, k.p, k.op, k.rc, (long long)k.jobs.size(), &r, st);
+                float2* from = k.spec_src >= 0 ? b->refpass[k.spec_src].work : nullptr;
+                rc = refpass_setup(b->ctx, k.p, k.op, k.rc, (long long)k.jobs.size(), &r, st, from);
                 if (rc != FFTSET_OK)
                     break;
                 b->refpass.push_back(r);
@@ -848,6 +928,7 @@ int fftset_batch_info(const struct fftset_batch* b, int* ntiles, int* nclasses, 
     if (ntiles) {
         int n = 0;
         for (auto& r : b->lds_r) n += r.count;
+        for (auto& r : b->lds_r1) n += r.count;
         for (int q = 0; q < 3; ++q)
             for (auto& r : b->fs_r[q]) n += r.count;
         *ntiles = n;
@@ -885,7 +966,7 @@ void fftset_batch_destroy(struct fftset_batch* b)
         (void)hipEventDestroy(b->fork);
     if (b->mid)
         (void)hipEventDestroy(b->mid);
-    if (b->dev_mem || b->work_pool || b->pass_work || !b->refpass.empty())
+    if (b->dev_mem || b->work_pool || b->pass_work || b->spec_pool || !b->refpass.empty())
         (void)hipDeviceSynchronize();
     for (RefPassClass& r : b->refpass) refpass_free(&r);
     if (b->dev_mem)
@@ -894,6 +975,8 @@ void fftset_batch_destroy(struct fftset_batch* b)
         (void)hipFree(b->work_pool);
     if (b->pass_work)
         (void)hipFree(b->pass_work);
+    if (b->spec_pool)
+        (void)hipFree(b->spec_pool);
     delete b;
 }
 

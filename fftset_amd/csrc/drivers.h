@@ -44,6 +44,10 @@ __device__ __forceinline__ void single_conv(int lt, const Load& ld, const Store&
 #pragma unroll
         for (int t = 0; t < R; ++t) v[t] = ld(t);
         DFT<R>::run(v);
+        if (bot.spec) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) bot.spec[k] = v[k];
+        }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const float2 K = bot.kern[k];
@@ -165,6 +169,10 @@ __device__ __forceinline__ void single_conv_pf(float2* pf, int lt, const BotConv
 {
     if (lt == 0) {
         DFT<R>::run(pf);
+        if (bot.spec) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) bot.spec[k] = pf[k];
+        }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const float2 K = bot.kern[k];

@@ -324,13 +324,21 @@ __device__ __forceinline__ void dif_bottom(const Tile& tl, int lt, int T, int s,
 // ------------------------------------------------------------ bottom operators
 // Convolution: w <- conj(w * K) (src/fftset_vec.c:847-867), then the first
 // DIT pass (m = 1: a plain forward DFT) in registers, back to the tile.
+// spec (optional): the signal's spectrum, written before the product in the
+// kernel-spectrum layout (what get_kernel of the signal gives), for the
+// kernel gradient of a later backward pass.
 struct BotConv {
     const float2* __restrict__ kern;
     bool conj_kern;      // true: w <- conj(w * conj(K)) (adjoint w.r.t. the signal)
     bool shared = false;  // one spectrum for every transform: cached loads
+    float2* __restrict__ spec = nullptr;
     template <int R, class Tile>
     __device__ __forceinline__ void apply(const Tile& tl, int b, int s, float2* v) const
     {
+        if (spec) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) stg(spec + k * s + b, v[k]);
+        }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const float2 K = shared ? kern[k * s + b] : ldg(kern + k * s + b);
@@ -346,9 +354,14 @@ struct BotConv {
 template <bool CONJ, bool SHARED>
 struct BotConvC {
     const float2* __restrict__ kern;
+    float2* __restrict__ spec = nullptr;  // optional signal spectrum (see BotConv)
     template <int R, class Tile>
     __device__ __forceinline__ void apply(const Tile& tl, int b, int s, float2* v) const
     {
+        if (spec) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) stg(spec + k * s + b, v[k]);
+        }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const float2 K = SHARED ? kern[k * s + b] : ldg(kern + k * s + b);

@@ -18,6 +18,8 @@ struct LdsArgs {
     float* out;
     const float* kern;
     long long in_st, out_st, kern_st;  // floats between consecutive transforms
+    float* spec;        // convolution: the signal's spectrum out (optional), stride spec_st
+    long long spec_st;
 };
 
 hipError_t launch_lds(int mod, int op, const LdsArgs& a, const ShapeD& sh, hipStream_t st);
@@ -42,6 +44,8 @@ struct FsArgs {
     float* out;
     const float* kern;
     long long in_st, out_st, kern_st;
+    float* spec;  // row step of a convolution: the signal's spectrum out (optional)
+    long long spec_st;
     // Per-transform pointers (ragged batches): when set, transform t reads
     // in_p[t] / kern_p[t] and writes out_p[t], and the strides are unused.
     const float* const* in_p;

@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(1024) k_fix(const LdsArgs a)
     } else {
         dif_mid<R1, NB1>(tl, lt, T, m1, s1, tw1);
         __syncthreads();
-        const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0};
+        const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0,
+                          a.spec ? (float2*)(a.spec + t0 * a.spec_st) : nullptr};
         dif_bottom<R2, NB2>(tl, lt, T, s2, bot);
         __syncthreads();
         dit_mid<R1, NB1>(tl, lt, T, m1, s1, tw1);

@@ -185,11 +185,13 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
                     StCx<true>{(float2*)out}(f, v);
             }
         } else if constexpr (KIND == RK_CONV) {
-            const BotConv bot{(const float2*)fs_kern(a, t0) + (long long)k1c * a.B, a.conj_kern != 0, fs_kern_shared(a)};
+            float2* spec = a.spec ? (float2*)(a.spec + t0 * a.spec_st) + (long long)k1c * a.B : nullptr;
+            const BotConv bot{(const float2*)fs_kern(a, t0) + (long long)k1c * a.B, a.conj_kern != 0, fs_kern_shared(a),
+                              spec};
             const StRowTw st{row_out, a.tfs + (long long)k1c * a.B};
             if constexpr (R0 > 0 && BOT >= 0)
                 run_conv_fix2<R0, R1>(a.c, tl, lt, ld, st,
-                                      BotConvC<(BOT & 1) != 0, (BOT & 2) != 0>{bot.kern});
+                                      BotConvC<(BOT & 1) != 0, (BOT & 2) != 0>{bot.kern, spec});
             else if constexpr (R0 > 0)
                 run_conv_fix2<R0, R1>(a.c, tl, lt, ld, st, bot);
             else
@@ -268,7 +270,8 @@ __global__ void __launch_bounds__(1024) k_rows(const FsArgs a)
                     StCx<true>{(float2*)out}(f, v);
             }
         } else if constexpr (KIND == RK_CONV) {
-            const BotConv bot{(const float2*)fs_kern(a, t) + (long long)k1 * a.B, a.conj_kern != 0, fs_kern_shared(a)};
+            const BotConv bot{(const float2*)fs_kern(a, t) + (long long)k1 * a.B, a.conj_kern != 0, fs_kern_shared(a),
+                              a.spec ? (float2*)(a.spec + t * a.spec_st) + (long long)k1 * a.B : nullptr};
             const StRowTw st{row_out, a.tfs + (long long)k1 * a.B};
             if (single) {
                 pf_single_conv<E>(a.c, pf, lt, bot, st);

@@ -89,7 +89,8 @@ __global__ void __launch_bounds__(lds_max_threads<E>()) k_lds(const LdsArgs a)
         } else if constexpr (OP == OP_KERN) {
             run_kern<E>(a.c, tl, lt, T, Io::ld(a, t0), (float2*)(a.out + t0 * a.out_st));
         } else {
-            const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0};
+            const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0,
+                          a.spec ? (float2*)(a.spec + t0 * a.spec_st) : nullptr};
             run_conv<E>(a.c, tl, lt, T, Io::ld(a, t0), Io::st(a, t0), bot);
         }
         return;
@@ -124,7 +125,8 @@ __global__ void __launch_bounds__(lds_max_threads<E>()) k_lds(const LdsArgs a)
             } else if constexpr (OP == OP_FWD || OP == OP_INV) {
                 pf_single_stockham<E>(a.c, pf, lt, Io::st(a, t0));
             } else {
-                const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0};
+                const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0,
+                          a.spec ? (float2*)(a.spec + t0 * a.spec_st) : nullptr};
                 pf_single_conv<E>(a.c, pf, lt, bot, Io::st(a, t0));
             }
             if (PERSIST && next < ntiles)
@@ -138,7 +140,8 @@ __global__ void __launch_bounds__(lds_max_threads<E>()) k_lds(const LdsArgs a)
             } else if constexpr (OP == OP_KERN) {
                 kern_tail<E>(a.c, tl, lt, T, (float2*)(a.out + t0 * a.out_st));
             } else {
-                const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0};
+                const BotConv bot{(const float2*)(a.kern + t0 * a.kern_st), OP == OP_GRAD_IN, a.kern_st == 0,
+                          a.spec ? (float2*)(a.spec + t0 * a.spec_st) : nullptr};
                 conv_tail<E>(a.c, tl, lt, T, Io::st(a, t0), bot);
             }
         }

@@ -60,6 +60,8 @@ extern "C" int fftset_fft_execute_noalloc(const struct fftset_fft* fft, int op, 
                                           size_t out_stride, const float* in, size_t in_stride, const float* kern,
                                           size_t kern_stride, void* stream)
 {
+    // FFTSET_OP_CONV_SAVE_SPECTRUM takes its spectrum output in the work
+    // argument, which this call does not have
     if (!fft || op < FFTSET_OP_FORWARD || op > FFTSET_OP_CONV_GRAD_KERNEL) {
         set_error(FFTSET_E_INVALID);
         return FFTSET_E_INVALID;

@@ -138,7 +138,7 @@ static bool lib_kern_freqs(const fftset_fft* p, std::vector<long long>* freq)
 
 // ------------------------------------------------------------ set-up / run
 int refpass_setup(Ctx* ctx, const fftset_fft* p, int op, const RClass& rc, long long ntrans, RefPassClass* out,
-                  hipStream_t st)
+                  hipStream_t st, float2* kref_from)
 {
     RefStructure rs;
     const unsigned L = p->lfft;
@@ -174,7 +174,10 @@ int refpass_setup(Ctx* ctx, const fftset_fft* p, int op, const RClass& rc, long 
         refpass_free(&k);
         return FFTSET_E_NOMEM;
     }
-    if (conv) {
+    if (conv && kref_from) {
+        k.kref = kref_from;
+        k.own_kref = false;
+    } else if (conv) {
         // the class's kernel spectra (library order) in this structure's
         // DIF order: kref[t][pos] = K_t[perm[pos]]
         std::vector<long long> lib;
@@ -206,6 +209,8 @@ int refpass_setup(Ctx* ctx, const fftset_fft* p, int op, const RClass& rc, long 
 
 void refpass_free(RefPassClass* k)
 {
+    if (!k->own_kref)
+        k->kref = nullptr;
     for (float2** b : {&k->work, &k->work2, &k->kref}) {
         if (*b)
             (void)hipFree(*b);

@@ -40,14 +40,17 @@ struct RefPassClass {
     float2* work = nullptr;   // ntrans * L
     float2* work2 = nullptr;  // ntrans * L (Stockham ping-pong)
     float2* kref = nullptr;   // ntrans * L (convolution: this structure's kernel spectra)
+    bool own_kref = true;     // false: kref is another class's get_kernel output
     int slot[8] = {0}, resid[8] = {0};
 };
 
 // Plans and allocates one class (transforms `rc`'s pointer arrays, op) and,
 // for convolutions, gathers the class's kernel spectra into the baseline's
-// own order. Returns an FFTSET_* code.
+// own order -- or, with kref_from, takes them from another class's
+// get_kernel output (already in that order, one per transform). Returns an
+// FFTSET_* code.
 int refpass_setup(Ctx* ctx, const fftset_fft* p, int op, const RClass& rc, long long ntrans, RefPassClass* out,
-                  hipStream_t st);
+                  hipStream_t st, float2* kref_from = nullptr);
 // Queues every sweep of the class on `st`.
 int refpass_run(const RefPassClass& k, hipStream_t st);
 void refpass_free(RefPassClass* k);

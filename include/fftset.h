@@ -122,6 +122,7 @@ int fftset_last_error(void);
 #define FFTSET_OP_CONV             3
 #define FFTSET_OP_CONV_GRAD_INPUT  4
 #define FFTSET_OP_CONV_GRAD_KERNEL 5
+#define FFTSET_OP_CONV_SAVE_SPECTRUM 6
 
 /* Uniform batch, stream-ordered (returns before completion): `count`
  * transforms of one plan; transform t reads in + t*in_stride, writes
@@ -133,6 +134,13 @@ int fftset_last_error(void);
  *                     in (the output gradient), with kern = get_kernel(h)
  *   CONV_GRAD_KERNEL: out = adjoint w.r.t. the kernel taps applied to kern
  *                     (the output gradient) with in = the signal x
+ *   CONV_SAVE_SPECTRUM: out = conv(in, kern), and work (required, stride
+ *                     work_stride) receives the signal's spectrum in the
+ *                     kernel-spectrum layout, i.e. get_kernel(in). The kernel
+ *                     gradient of a later backward pass is then one
+ *                     CONV_GRAD_INPUT of the output gradient with that
+ *                     spectrum as kern, instead of CONV_GRAD_KERNEL's
+ *                     get_kernel of the signal plus that pass.
  * work (may be NULL) provides count*work_stride floats of scratch for the
  * operations that need it; when NULL the library uses its own, one buffer
  * per stream (so calls on different streams never share it). A buffer that
@@ -210,8 +218,13 @@ struct fftset_batch;
 struct fftset_batch *fftset_batch_create(struct fftset *fc);
 
 /* Appends one transform: op is FFTSET_OP_FORWARD, _INVERSE, _GET_KERNEL,
- * _CONV or _CONV_GRAD_INPUT, with the buffers of the matching drop-in call
- * (kern for the convolutions). Returns the transform's index or an error. */
+ * _CONV, _CONV_GRAD_INPUT or _CONV_GRAD_KERNEL, with the buffers of
+ * fftset_fft_execute for that op (kern for the convolutions; for
+ * _CONV_GRAD_KERNEL in = the signal, kern = the output gradient). A kernel
+ * gradient runs as a get_kernel of its signal into a spectrum the batch
+ * owns, then, after every other transform of the launch, the signal
+ * adjoint of its output gradient against that spectrum. Returns the
+ * transform's index or an error. */
 int fftset_batch_add(struct fftset_batch *b, const struct fftset_fft *fft, int op, float *out, const float *in,
                      const float *kern);
 
@@ -240,7 +253,7 @@ unsigned fftset_baseline_chain(const struct fftset_modulation *modulation, unsig
                                unsigned *radices, unsigned max);
 
 /* Algorithmic bytes of one launch: 16 L per forward / inverse / get_kernel,
- * 24 L per convolution. */
+ * 24 L per convolution, signal adjoint or kernel gradient. */
 long long fftset_batch_bytes(const struct fftset_batch *b);
 
 /* Tile count, class count, LDS bytes per workgroup and the number of

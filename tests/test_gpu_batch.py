@@ -221,6 +221,76 @@ def test_reference_pass_baseline_computes_the_same_transforms(fs):
     b.destroy()
 
 
+GK_SPEC = [
+    (FO, 768, fftset_amd.OP_CONV_GRAD_KERNEL, 5), (CX, 360, fftset_amd.OP_CONV_GRAD_KERNEL, 4),
+    (FO, 2048, fftset_amd.OP_CONV_GRAD_KERNEL, 3), (FO, 61440, fftset_amd.OP_CONV_GRAD_KERNEL, 2),
+    (FO, 15360, fftset_amd.OP_CONV_GRAD_KERNEL, 1), (CX, 61440, fftset_amd.OP_CONV_GRAD_KERNEL, 1),
+    (FO, 768, fftset_amd.OP_CONV, 4), (FO, 768, fftset_amd.OP_GET_KERNEL, 2), (CX, 4608, fftset_amd.OP_FORWARD, 3),
+    (FO, 61440, fftset_amd.OP_CONV, 1),
+]
+
+
+def _grad_kernel64(jb):
+    # dLoss/dh for output gradient g = the job's h-array and signal x
+    x, g, L = jb["x"], jb["h"], jb["L"]
+    if jb["mod"] == FO:
+        return fp64.fo_conv_grads64(g[None], x[None], np.zeros_like(x)[None])[1][0]
+    X = np.fft.fft(fp64.as_complex(x))
+    G = np.fft.fft(fp64.as_complex(g))
+    return fp64.as_interleaved(np.fft.ifft(G * np.conj(X)) * L)
+
+
+def test_kernel_gradient_jobs_in_ragged_batches(fs):
+    # FFTSET_OP_CONV_GRAD_KERNEL in a batch (in = signal, kern = output
+    # gradient): its get_kernel stage and its adjoint stage run in one
+    # launch; outputs match fftset_fft_execute and FP64, over graph replays
+    # and every baseline
+    rng = np.random.default_rng(33)
+    jobs = _make_jobs(rng, GK_SPEC)
+    b = fs.batch()
+    for jb in jobs:
+        f = fs.create_fft(jb["mod"], jb["L"])
+        jb["f"] = f
+        jb["dx"] = torch.from_numpy(jb["x"]).cuda()
+        jb["dy"] = torch.full_like(jb["dx"], float("nan"))
+        jb["dk"] = None
+        if jb["op"] == fftset_amd.OP_CONV_GRAD_KERNEL:
+            jb["dk"] = torch.from_numpy(jb["h"]).cuda()  # the output gradient
+        elif jb["op"] == fftset_amd.OP_CONV:
+            jb["dk"] = torch.empty_like(jb["dx"])
+            f.conv_get_kernel(jb["dk"], torch.from_numpy(jb["h"]).cuda())
+        b.add(f, jb["op"], jb["dy"], jb["dx"], jb["dk"])
+    b.commit()
+    expect = []
+    for jb in jobs:
+        ref = torch.full_like(jb["dx"], float("nan"))
+        jb["f"].execute(jb["op"], 1, ref, jb["dx"], kern=jb["dk"])
+        torch.cuda.synchronize()
+        expect.append(ref.cpu().numpy())
+    gk_bytes = sum(24 * jb["L"] for jb in jobs if jb["op"] == fftset_amd.OP_CONV_GRAD_KERNEL)
+    assert b.bytes == gk_bytes + sum((24 if jb["op"] == fftset_amd.OP_CONV else 16) * jb["L"]
+                                     for jb in jobs if jb["op"] != fftset_amd.OP_CONV_GRAD_KERNEL)
+    for mode in ("fused", "fused", fftset_amd.BASELINE_PER_GROUP, fftset_amd.BASELINE_PER_TRANSFORM,
+                 fftset_amd.BASELINE_PER_PASS, fftset_amd.BASELINE_REFERENCE_PASSES):
+        for jb in jobs:
+            jb["dy"].fill_(float("nan"))
+        if mode == "fused":
+            b.launch()
+        else:
+            b.launch_baseline(mode)
+        torch.cuda.synchronize()
+        for jb, ref in zip(jobs, expect):
+            if jb["op"] == fftset_amd.OP_GET_KERNEL and mode == fftset_amd.BASELINE_REFERENCE_PASSES:
+                continue  # that baseline keeps spectra in its own order
+            tag = (mode, jb["mod"], jb["L"], jb["op"])
+            y = jb["dy"].cpu().numpy()
+            assert np.all(np.isfinite(y)), tag
+            assert fp64.rel_rms(y, ref) <= (2e-6 if mode == fftset_amd.BASELINE_REFERENCE_PASSES else 1e-6), tag
+            if jb["op"] == fftset_amd.OP_CONV_GRAD_KERNEL:
+                assert fp64.rel_rms(y, _grad_kernel64(jb)) <= 1e-6 * np.log2(jb["L"]), tag
+    b.destroy()
+
+
 def test_batch_four_step_classes_equal_drop_in_calls(fs):
     # four-step classes run the drop-in four-step kernels through the
     # class's pointer arrays: outputs equal single-transform drop-in calls bit
@@ -273,7 +343,9 @@ def test_batch_rejects_bad_use(fs):
     f = fs.create_fft(CX, 64)
     x = torch.zeros(128, device="cuda")
     with pytest.raises(fftset_amd.FFTSetError):
-        b.add(f, fftset_amd.OP_CONV_GRAD_KERNEL, x, x, x)  # two-phase op: not batchable
+        b.add(f, fftset_amd.OP_CONV_GRAD_KERNEL, x, x, None)  # needs the output gradient
+    with pytest.raises(fftset_amd.FFTSetError):
+        b.add(f, fftset_amd.OP_CONV_SAVE_SPECTRUM, x, x, x)  # execute() only
     with pytest.raises(fftset_amd.FFTSetError):
         b.add(f, fftset_amd.OP_CONV, x, x, None)  # conv needs a kernel spectrum
     b.add(f, fftset_amd.OP_FORWARD, x, x)

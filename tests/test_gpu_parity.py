@@ -458,3 +458,37 @@ def test_one_workgroup_four_step_equals_split_launches(fs, mod, op):
     else:
         return
     assert np.max(fp64.rel_rms(fused[[0, n - 1]], ref)) <= fp64_tol(L)
+
+
+@pytest.mark.parametrize("mod,L", [(FO, 8), (FO, 120), (CX, 360), (FO, 2048), (FO, 4096), (CX, 20480),
+                                   (FO, 61440)])
+def test_conv_save_spectrum(fs, mod, L):
+    # FFTSET_OP_CONV_SAVE_SPECTRUM: the same output as CONV (the same kernel,
+    # so bit-identical), the signal's spectrum as get_kernel gives it, and
+    # the kernel gradient as CONV_GRAD_INPUT with that spectrum
+    import torch
+    f = fs.create_fft(mod, L)
+    n = 6
+    rng = np.random.default_rng(L + 11)
+    x = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
+    h = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
+    g = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
+    K, X, Xk = (torch.empty_like(x) for _ in range(3))
+    y0, y1, gh0, gh1 = (torch.full_like(x, float("nan")) for _ in range(4))
+    f.execute(fftset_amd.OP_GET_KERNEL, n, K, h)
+    f.execute(fftset_amd.OP_GET_KERNEL, n, Xk, x)
+    f.execute(fftset_amd.OP_CONV, n, y0, x, kern=K)
+    f.execute(fftset_amd.OP_CONV_SAVE_SPECTRUM, n, y1, x, kern=K, work=X)
+    f.execute(fftset_amd.OP_CONV_GRAD_KERNEL, n, gh0, x, kern=g)
+    f.execute(fftset_amd.OP_CONV_GRAD_INPUT, n, gh1, g, kern=X)
+    torch.cuda.synchronize()
+    assert np.array_equal(y0.cpu().numpy(), y1.cpu().numpy())
+    assert np.max(fp64.rel_rms(X.cpu().numpy(), Xk.cpu().numpy())) <= 1e-6
+    gh = gh1.cpu().numpy()
+    assert np.max(fp64.rel_rms(gh, gh0.cpu().numpy())) <= 1e-6
+    if mod == FO:
+        xs, hs, gs = x.cpu().numpy()[[0, n - 1]], h.cpu().numpy()[[0, n - 1]], g.cpu().numpy()[[0, n - 1]]
+        _, gh_ref = fp64.fo_conv_grads64(gs, xs, hs)
+        assert np.max(fp64.rel_rms(gh[[0, n - 1]], gh_ref)) <= fp64_tol(L)
+    with pytest.raises(fftset_amd.FFTSetError):  # the spectrum output is required
+        f.execute(fftset_amd.OP_CONV_SAVE_SPECTRUM, n, y1, x, kern=K)
