@@ -276,6 +276,23 @@ def workload_block(torch, fs, rng_np, args):
     return kernels, bytes_per, cfg, check, cpu_sample
 
 
+def host_cpu():
+    """What the CPU baseline ran: the scalar (v1f) C restatement of the
+    reference's path, built portable (-O2, no -march, no SIMD), on `cores`
+    threads of the host named here."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"code": "scalar port (v1f restatement, gcc -O2, no SIMD)", "host_cpu": model,
+            "host_logical_cores": os.cpu_count()}
+
+
 def cpu_baseline_block(x_dev, h_dev, L, b, seconds):
     """Oracle FREQOFFSETREAL convolution of overlap-save frames on the host
     cores, one frame per thread, over the first blocks of channel 0 (frames
@@ -610,7 +627,7 @@ def main():
             out["target_5x"] = {"baseline": "per_reference_pass_launches",
                                 "speedup": out["speedup_vs_baselines"]["per_reference_pass_launches"]}
     if not args.no_cpu and cpu_sample is not None:
-        out["cpu_baseline"] = cpu_sample(args.cpu_seconds)
+        out["cpu_baseline"] = dict(cpu_sample(args.cpu_seconds), **host_cpu())
     fs.destroy()
     print(json.dumps(out))
 

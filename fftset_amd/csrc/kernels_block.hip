@@ -96,11 +96,74 @@ __global__ void __launch_bounds__(1024) k_block(const BlockArgs a)
     run_conv<E>(a.c, tl, lt, T, ld, st, bot);
 }
 
-// The same frame convolution with a four-pass chain [R0, R1, R2, R3] and the
-// lanes per transform as template constants (strides, bounds and LDS offsets
-// fold at compile time), for the tuned chain of the block size.
-template <int R0, int R1, int R2, int R3, int T>
-__global__ void __launch_bounds__(1024) k_block_fix(const BlockArgs a)
+// Frame access for the one-frame-per-workgroup kernel. The frame's first
+// sample (x + s0) and first output (y + s0) are uniform across the
+// workgroup, so every access is an SGPR base plus a 32-bit lane offset (no
+// per-lane 64-bit address arithmetic). Readable frame positions are
+// [lo, hi); FULL frames (lo = 0, hi = N: every frame of a channel but its
+// first and last) read and write without range checks.
+__device__ __forceinline__ const float& at32(const float* p, int i)
+{
+    return *(const float*)((const char*)p + (unsigned)i * 4u);
+}
+
+template <bool FOLD, bool FULL>
+struct LdFrameU {
+    static constexpr bool kFoFold = FOLD;
+    const float* __restrict__ xb;  // x + s0: only positions in [lo, hi) are read
+    int lo, hi, L;
+    const float2* __restrict__ tmod;
+    const float2* __restrict__ tfo;
+    __device__ __forceinline__ float sample(int s) const
+    {
+        if constexpr (FULL) {
+            return at32(xb, s);
+        } else {
+            const bool in = (unsigned)(s - lo) < (unsigned)(hi - lo);
+            const float v = at32(xb, in ? s : lo);  // lo < hi holds for every frame
+            return in ? v : 0.f;
+        }
+    }
+    __device__ __forceinline__ float2 raw(int e) const { return make_float2(sample(e), -sample(L + e)); }
+    __device__ __forceinline__ float2 operator()(int e) const
+    {
+        const float re = sample(e), im = sample(L + e);
+        const float2 w = tmod[e];
+        return make_float2(re * w.x + im * w.y, re * w.y - im * w.x);
+    }
+};
+
+template <bool FOLD, bool FULL>
+struct StFrameU {
+    static constexpr bool kFoFold = FOLD;
+    float* __restrict__ yb;  // y + s0: positions p >= skip (and p < hi) are outputs
+    int hi, L, skip;
+    const float2* __restrict__ tmod;
+    const float2* __restrict__ tfo;
+    __device__ __forceinline__ void put(int p, float v) const
+    {
+        if (p >= skip && (FULL || p < hi))
+            __builtin_nontemporal_store(v, (float*)((char*)yb + (unsigned)p * 4u));
+    }
+    __device__ __forceinline__ void put_demod(int f, float2 v) const
+    {
+        put(f, v.x);
+        put(L + f, v.y);
+    }
+    __device__ __forceinline__ void operator()(int f, float2 v) const
+    {
+        const float2 w = tmod[f];
+        put(f, v.x * w.x - v.y * w.y);
+        put(L + f, v.x * w.y + v.y * w.x);
+    }
+};
+
+// The frame convolution with a four-pass chain [R0, R1, R2, R3] and T lanes
+// as template constants (strides, bounds and LDS offsets fold at compile
+// time), one frame per workgroup of exactly T lanes.
+template <int R0, int R1, int R2, int R3, int T, bool FULL>
+__device__ __forceinline__ void block_fix_frame(const BlockArgs& a, const TileContig& tl, int lt, const float* xb,
+                                                float* yb, int lo, int hi)
 {
     constexpr int E = 16;
     constexpr int n = R0 * R1 * R2 * R3;
@@ -110,24 +173,9 @@ __global__ void __launch_bounds__(1024) k_block_fix(const BlockArgs a)
     constexpr int NB1 = (E / R1) > 0 ? E / R1 : 1;
     constexpr int NB2 = (E / R2) > 0 ? E / R2 : 1;
     constexpr int NB3 = (E / R3) > 0 ? E / R3 : 1;
-    constexpr int kIdle = 1 << 28;
-    extern __shared__ float2 smem[];
-    const int G = a.G;
-    const int g = threadIdx.x / T;
-    const int lt0 = threadIdx.x - g * T;
-    const bool in_tile = g < G;
-    const TileContig tl{smem + (in_tile ? g : 0) * lds_len(n)};
-
-    const long long tr = (long long)blockIdx.x * G + g;
-    const bool ok = in_tile && tr < a.ntrans;
-    const int lt = ok ? lt0 : kIdle;
-    const long long t0 = ok ? tr : 0;
-    const long long c = t0 / a.nblocks;
-    const long long i = t0 - c * a.nblocks;
-    const long long s0 = i * a.b - a.skip;
-    // folding both sides exceeds 128 VGPRs here; the closing pass fits
-    const LdFrame<false> ld{a.in + c * a.in_st, s0, a.lo, a.n, n, a.tmod, a.tfo};
-    const StFrame<true> st{a.out + c * a.out_st, s0, a.n, n, a.skip, a.tmod, a.tfo};
+    // modulation folded into the first pass, demodulation into the last
+    const LdFrameU<true, FULL> ld{xb, lo, hi, n, a.tmod, a.tfo};
+    const StFrameU<true, FULL> st{yb, hi, n, a.skip, a.tmod, a.tfo};
     const BotConv bot{(const float2*)a.kern, false, true};
     const float2* tw0 = a.c.p[0].tw;
     const float2* tw1 = a.c.p[1].tw;
@@ -148,6 +196,26 @@ __global__ void __launch_bounds__(1024) k_block_fix(const BlockArgs a)
     dit_last<R0, NB0>(tl, lt, T, m0, tw0, st);
 }
 
+template <int R0, int R1, int R2, int R3, int T>
+__global__ void __launch_bounds__(T) k_block_fix(const BlockArgs a)
+{
+    constexpr int N = 2 * R0 * R1 * R2 * R3;
+    extern __shared__ float2 smem[];
+    const TileContig tl{smem};
+    const int lt = threadIdx.x;
+    const long long t = blockIdx.x;
+    const long long c = t / a.nblocks;
+    const long long s0 = (t - c * a.nblocks) * a.b - a.skip;
+    const int lo = (int)(a.lo > s0 ? a.lo - s0 : 0);
+    const int hi = (int)(a.n - s0 < N ? a.n - s0 : N);
+    const float* xb = a.in + c * a.in_st + s0;
+    float* yb = a.out + c * a.out_st + s0;
+    if (lo == 0 && hi == N)
+        block_fix_frame<R0, R1, R2, R3, T, true>(a, tl, lt, xb, yb, lo, hi);
+    else
+        block_fix_frame<R0, R1, R2, R3, T, false>(a, tl, lt, xb, yb, lo, hi);
+}
+
 static bool chain_is4(const ChainD& c, int r0, int r1, int r2, int r3)
 {
     return c.np == 4 && c.p[0].R == r0 && c.p[1].R == r1 && c.p[2].R == r2 && c.p[3].R == r3;
@@ -165,7 +233,8 @@ hipError_t launch_block(const BlockArgs& a, const ShapeD& sh, hipStream_t st)
     auto fn = k_block<16>;
     // FREQOFFSETREAL 8192 = [16,8,8,8] (the tuned chain) has a specialised
     // kernel; the kernel selection can keep the generic one
-    const bool fix = !generic_forced() && a.L == 8192 && sh.T == 512 && chain_is4(a.c, 16, 8, 8, 8);
+    const bool fix = !generic_forced() && a.L == 8192 && sh.T == 512 && sh.G == 1 && sh.threads == 512 &&
+                     chain_is4(a.c, 16, 8, 8, 8);
     if (fix)
         fn = k_block_fix<16, 8, 8, 8, 512>;
     count_launch(fix);

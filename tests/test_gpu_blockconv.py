@@ -60,6 +60,51 @@ def test_blockconv_matches_fp64(fs, oracle, k, max_block, extra):
     bc.destroy()
 
 
+@pytest.mark.parametrize("case", ["partial_single_frame", "exact_blocks", "history_strided", "ragged_tail"])
+def test_blockconv_specialised_frame_bounds(case):
+    # the specialised [16,8,8,8] frame kernel reads frames inside the signal
+    # without range checks and the first/last frames of a channel with them;
+    # every case is checked against FP64 direct convolution, and the padding
+    # past each channel's signal must stay untouched
+    import os
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    tune = os.path.join(os.path.dirname(fftset_amd.__file__), "tune_mi355x.txt")
+    s = fftset_amd.FFTSet(tune_file=tune)
+    k = 1024
+    h = _taps(k, 29)
+    bc = s.blockconv(torch.from_numpy(h).cuda(), 0)
+    b = bc.block_size
+    assert bc.complex_bins == 8192
+    n = {"partial_single_frame": b // 2, "exact_blocks": 2 * b, "history_strided": 2 * b + 77,
+         "ragged_tail": 4 * b + 1}[case]
+    C, pad = 3, 96
+    rng = np.random.default_rng({"partial_single_frame": 1, "exact_blocks": 2, "history_strided": 3, "ragged_tail": 4}[case])
+    x = rng.standard_normal((C, n)).astype(np.float32)
+    stride = n + pad
+    xs = torch.zeros((C, stride), device="cuda")
+    xs[:, :n] = torch.from_numpy(x).cuda()
+    y = torch.full((C, stride), float("nan"), device="cuda")
+    fftset_amd.kernel_counts()
+    if case == "history_strided":
+        # two calls; the second reads the first part's last k-1 samples as history
+        n1 = b + 333
+        bc.run(y, xs, nsamples=n1, channels=C, in_stride=stride, out_stride=stride)
+        bc.run(y.data_ptr() + 4 * n1, xs.data_ptr() + 4 * n1, nsamples=n - n1, channels=C, in_stride=stride,
+               out_stride=stride, flags=fftset_amd.BLOCKCONV_HISTORY)
+        assert fftset_amd.kernel_counts() == (2, 0)
+    else:
+        bc.run(y, xs, nsamples=n, channels=C, in_stride=stride, out_stride=stride)
+        assert fftset_amd.kernel_counts() == (1, 0)
+    torch.cuda.synchronize()
+    yv = y.cpu().numpy()
+    for c in range(C):
+        assert fp64.rel_rms(yv[c, :n], _fir64(x[c], h)) <= 1e-6 * np.log2(2 * bc.complex_bins), (case, c)
+        assert np.all(np.isnan(yv[c, n:])), (case, c)
+    bc.destroy()
+    s.destroy()
+
+
 def test_blockconv_history_and_strides(fs):
     # a signal filtered in two calls, the second reading the first's tail as
     # history, equals one call over the whole signal
