@@ -43,6 +43,37 @@ __device__ __forceinline__ float2 ldg(const float2* p)
 __device__ __forceinline__ float ldg(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void stg(float2* p, float2 v) { __builtin_nontemporal_store(fsa_f2v{v.x, v.y}, (fsa_f2v*)p); }
 __device__ __forceinline__ void stg(float* p, float v) { __builtin_nontemporal_store(v, p); }
+// Element i of p with its byte offset formed in 32 bits (0 <= i and
+// i * sizeof(T) < 2^32). For a p that is uniform across the wave (a
+// twiddle or modulation table) the access is an SGPR base plus a 32-bit
+// lane offset, where p + i costs 64-bit address arithmetic per lane. For a
+// per-lane p (one transform's signal among several in a workgroup) plain
+// p + i is cheaper: the compiler folds constant offsets into one 64-bit
+// base.
+template <class T>
+__device__ __forceinline__ const T* off32(const T* p, int i)
+{
+    return (const T*)((const char*)p + (unsigned)i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T* off32(T* p, int i)
+{
+    return (T*)((char*)p + (unsigned)i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ const T& at32(const T* p, int i)
+{
+    return *off32(p, i);
+}
+// p[uni + lane] for a wave-uniform p and uni (a twiddle table and a pass's
+// compile-time or uniform offset) and a per-lane lane >= 0: the uniform part
+// goes into the scalar base, so the twiddles of one butterfly share one
+// 32-bit lane offset.
+template <class T>
+__device__ __forceinline__ const T& at2(const T* p, int uni, int lane)
+{
+    return *off32(p + uni, lane);
+}
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // -i * a and +i * a
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }

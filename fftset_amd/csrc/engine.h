@@ -89,7 +89,7 @@ __device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m,
             if (u < m) {
                 DFT<R>::run(v[q]);
 #pragma unroll
-                for (int k = 0; k < R; ++k) tl.at(u + k * m) = cmul(v[q][k], ld.tfo[k * m + u]);
+                for (int k = 0; k < R; ++k) tl.at(u + k * m) = cmul(v[q][k], at2(ld.tfo, k * m, u));
             }
         }
         return;
@@ -108,7 +108,7 @@ __device__ __forceinline__ void pass_first(const Tile& tl, int lt, int T, int m,
         if (u < m) {
             DFT<R>::run(v[q]);
 #pragma unroll
-            for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[(k - 1) * m + u]);
+            for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], at2(tw, (k - 1) * m, u));
 #pragma unroll
             for (int k = 0; k < R; ++k) tl.at(u + k * m) = v[q][k];
         }
@@ -142,7 +142,7 @@ __device__ __forceinline__ void first_compute(float2* pf, const Tile& tl, int lt
             float2* v = pf + q * R;
             DFT<R>::run(v);
 #pragma unroll
-            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], tw[(k - 1) * m + u]);
+            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], at2(tw, (k - 1) * m, u));
 #pragma unroll
             for (int k = 0; k < R; ++k) tl.at(u + k * m) = v[k];
         }
@@ -169,7 +169,7 @@ __device__ __forceinline__ void stock_mid(const Tile& tl, int lt, int T, int m, 
             DFT<R>::run(v[q]);
             if (m > 1) {
 #pragma unroll
-                for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], tw[(k - 1) * m + j]);
+                for (int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], at2(tw, (k - 1) * m, j));
             }
         }
     }
@@ -234,7 +234,7 @@ __device__ __forceinline__ void dif_mid(const Tile& tl, int lt, int T, int m, in
             for (int t = 0; t < R; ++t) v[t] = tl.at(base + t * m);
             DFT<R>::run(v);
 #pragma unroll
-            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], tw[(k - 1) * m + j]);
+            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], at2(tw, (k - 1) * m, j));
 #pragma unroll
             for (int k = 0; k < R; ++k) tl.at(base + k * m) = v[k];
         }
@@ -256,7 +256,7 @@ __device__ __forceinline__ void dit_mid(const Tile& tl, int lt, int T, int m, in
             float2 v[R];
             v[0] = tl.at(base);
 #pragma unroll
-            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(base + t * m), tw[(t - 1) * m + j]);
+            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(base + t * m), at2(tw, (t - 1) * m, j));
             DFT<R>::run(v);
 #pragma unroll
             for (int k = 0; k < R; ++k) tl.at(base + k * m) = v[k];
@@ -276,7 +276,7 @@ __device__ __forceinline__ void dit_last(const Tile& tl, int lt, int T, int m, c
             if (u < m) {
                 float2 v[R];
 #pragma unroll
-                for (int t = 0; t < R; ++t) v[t] = cmul(tl.at(u + t * m), st.tfo[t * m + u]);
+                for (int t = 0; t < R; ++t) v[t] = cmul(tl.at(u + t * m), at2(st.tfo, t * m, u));
                 DFT<R>::run(v);
                 static_for<R>([&](auto Kk) {
                     constexpr int k = decltype(Kk)::value;
@@ -293,7 +293,7 @@ __device__ __forceinline__ void dit_last(const Tile& tl, int lt, int T, int m, c
             float2 v[R];
             v[0] = tl.at(u);
 #pragma unroll
-            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(u + t * m), tw[(t - 1) * m + u]);
+            for (int t = 1; t < R; ++t) v[t] = cmul(tl.at(u + t * m), at2(tw, (t - 1) * m, u));
             DFT<R>::run(v);
 #pragma unroll
             for (int k = 0; k < R; ++k) st(u + k * m, v[k]);
@@ -394,7 +394,7 @@ struct LdFoMod {
     __device__ __forceinline__ float2 operator()(int e) const
     {
         const float re = ldg(x + e), im = ldg(x + L + e);
-        const float2 w = tmod[e];
+        const float2 w = at32(tmod, e);
         return make_float2(re * w.x + im * w.y, re * w.y - im * w.x);
     }
 };
@@ -455,7 +455,7 @@ struct StFoDemod {
     const float2* __restrict__ tmod;
     __device__ __forceinline__ void operator()(int f, float2 v) const
     {
-        const float2 w = tmod[f];
+        const float2 w = at32(tmod, f);
         stg(y + f, v.x * w.x - v.y * w.y);
         stg(y + L + f, v.x * w.y + v.y * w.x);
     }

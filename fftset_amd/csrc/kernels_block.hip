@@ -102,11 +102,6 @@ __global__ void __launch_bounds__(1024) k_block(const BlockArgs a)
 // per-lane 64-bit address arithmetic). Readable frame positions are
 // [lo, hi); FULL frames (lo = 0, hi = N: every frame of a channel but its
 // first and last) read and write without range checks.
-__device__ __forceinline__ const float& at32(const float* p, int i)
-{
-    return *(const float*)((const char*)p + (unsigned)i * 4u);
-}
-
 template <bool FOLD, bool FULL>
 struct LdFrameU {
     static constexpr bool kFoFold = FOLD;

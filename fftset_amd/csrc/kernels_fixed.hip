@@ -12,7 +12,12 @@
 
 namespace fsa {
 
-template <int MOD, int OP, int R0, int R1, int R2, int T>
+// ONE: one transform per workgroup of exactly T lanes (G = 1). The
+// transform index is then blockIdx.x, uniform across the workgroup, so its
+// signal, output and kernel bases live in SGPRs and every access is that
+// base plus a 32-bit lane offset; without it each lane carries 64-bit
+// addresses.
+template <int MOD, int OP, int R0, int R1, int R2, int T, bool ONE>
 __global__ void __launch_bounds__(1024) k_fix(const LdsArgs a)
 {
     constexpr int E = 16;
@@ -24,13 +29,13 @@ __global__ void __launch_bounds__(1024) k_fix(const LdsArgs a)
     constexpr int kIdle = 1 << 28;
     extern __shared__ float2 smem[];
     using Io = LdsIo<MOD, OP>;
-    const int G = a.G;
-    const int g = threadIdx.x / T;
+    const int G = ONE ? 1 : a.G;
+    const int g = ONE ? 0 : threadIdx.x / T;
     const int lt0 = threadIdx.x - g * T;
-    const bool in_tile = g < G;
+    const bool in_tile = ONE || g < G;
     const TileContig tl{smem + (in_tile ? g : 0) * lds_len(n)};
     const long long tr = (long long)blockIdx.x * G + g;
-    const bool ok = in_tile && tr < a.ntrans;
+    const bool ok = ONE || (in_tile && tr < a.ntrans);  // ONE: the grid is ntrans workgroups
     const int lt = ok ? lt0 : kIdle;
     const long long t0 = ok ? tr : 0;
     const float2* tw0 = a.c.p[0].tw;
@@ -65,7 +70,9 @@ static hipError_t launch_fix(const LdsArgs& a, const ShapeD& sh, hipStream_t st)
     const long long tiles = (a.ntrans + sh.G - 1) / sh.G;
     if (tiles <= 0)
         return hipSuccess;
-    auto fn = k_fix<MOD, OP, R0, R1, R2, T>;
+    if (tiles > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    auto fn = (sh.G == 1 && sh.threads == T) ? k_fix<MOD, OP, R0, R1, R2, T, true> : k_fix<MOD, OP, R0, R1, R2, T, false>;
     if (sh.lds_bytes > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, sh.lds_bytes);
     (void)hipGetLastError();  // report this launch, not an earlier failed call
