@@ -84,6 +84,8 @@ def load_library():
     lib.fftset_fft_execute_noalloc.restype = ctypes.c_int
     lib.fftset_fft_complex_bins.argtypes = [vp]
     lib.fftset_fft_complex_bins.restype = u
+    lib.fftset_table_bytes.argtypes = [vp]
+    lib.fftset_table_bytes.restype = ctypes.c_ulonglong
     lib.fftset_fft_radices.argtypes = [vp, ctypes.POINTER(u), u]
     lib.fftset_fft_radices.restype = u
     lib.fftset_baseline_chain.argtypes = [vp, u, ctypes.POINTER(u), u]
@@ -469,6 +471,11 @@ class FFTSet:
         if rc != OK:
             raise FFTSetError(rc, f"autotune({modulation}, {complex_bins})")
         return ms.value
+
+    @property
+    def table_bytes(self):
+        """HBM bytes held by the set's twiddle and modulation tables."""
+        return int(load_library().fftset_table_bytes(self.address))
 
     def tune_save(self, path):
         rc = load_library().fftset_tune_save(self.address, os.fsencode(path))

@@ -79,6 +79,17 @@ const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipSt
     return t.dev;
 }
 
+void release_tables_after(Ctx* ctx, size_t mark)
+{
+    for (size_t i = mark; i < ctx->tables.size(); ++i) {
+        const Table& t = ctx->tables[i];
+        ctx->table_bytes -= sizeof(float2) * (size_t)t.rows * (size_t)t.cols;
+        (void)hipFree(t.dev);
+    }
+    if (ctx->tables.size() > mark)
+        ctx->tables.resize(mark);
+}
+
 // Attaches the per-pass tables W_{R m}^{jk} ([k-1][j] layout) to a chain.
 bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st)
 {
@@ -736,6 +747,11 @@ int fftset_last_error(void)
 }
 
 unsigned fftset_fft_complex_bins(const struct fftset_fft* fft) { return fft ? fft->lfft : 0; }
+
+unsigned long long fftset_table_bytes(const struct fftset* fc)
+{
+    return (fc && fc->impl) ? (unsigned long long)((const Ctx*)fc->impl)->table_bytes : 0;
+}
 
 unsigned fftset_fft_radices(const struct fftset_fft* fft, unsigned* radices, unsigned max)
 {

@@ -189,6 +189,20 @@ int fftset_autotune(struct fftset* fc, const struct fftset_modulation* modulatio
         }
     int rc = use_device(ctx);
     hipStream_t st = thread_stream(ctx->device);
+    // Candidate plans take their tables from the set's arena; the ones made
+    // only for candidates are freed on every way out (plans created after
+    // tuning build the winner's tables anew).
+    struct CandidateTables {
+        Ctx* ctx;
+        size_t mark;
+        hipStream_t st;
+        ~CandidateTables()
+        {
+            if (st)
+                (void)hipStreamSynchronize(st);
+            release_tables_after(ctx, mark);
+        }
+    } cand_tables{ctx, ctx->tables.size(), st};
     if (rc != FFTSET_OK || !st) {
         set_error(FFTSET_E_DEVICE);
         return FFTSET_E_DEVICE;

@@ -111,6 +111,9 @@ hipStream_t thread_stream(int device);
 int use_device(const Ctx* ctx);
 void set_error(int code);
 const float2* get_table(Ctx* ctx, long long N, int a0, int rows, int cols, hipStream_t st, int rs = 1);
+// Frees the tables created after the arena held `mark` of them (the caller
+// has synchronised every stream that used them and no plan refers to them).
+void release_tables_after(Ctx* ctx, size_t mark);
 bool attach_twiddles(Ctx* ctx, ChainD* c, hipStream_t st);
 // Four-step split of L whose column and row tiles fit max_threads lanes;
 // prefer_A > 0 forces that column length when it is valid.

@@ -332,6 +332,10 @@ void fftset_kernel_counts(unsigned long long *specialised, unsigned long long *g
 
 /* Plan introspection (for tests, benchmarks and the autotuner). */
 unsigned fftset_fft_complex_bins(const struct fftset_fft *fft);
+/* HBM bytes held by the set's shared twiddle and modulation tables (no
+ * counterpart in the reference, whose tables live in host memory). Tables
+ * made only for fftset_autotune's candidates are freed before it returns. */
+unsigned long long fftset_table_bytes(const struct fftset *fc);
 /* Writes at most `max` radices of the plan's LDS chain; returns the count
  * (for four-step plans: the chains of the column and row transforms, with
  * a 0 separating them). */

@@ -23,9 +23,12 @@ def test_autotune_then_plan(oracle, tmp_path, mod, L):
         pytest.skip("no GPU")
     fs = fftset_amd.FFTSet()
     try:
+        before = fs.table_bytes
         ms = fs.autotune(mod, L)
         assert ms > 0
+        assert fs.table_bytes == before  # the candidates' tables are freed
         f = fs.create_fft(mod, L)
+        assert fs.table_bytes > before
         radices = f.radices()
         with pytest.raises(fftset_amd.FFTSetError):
             fs.autotune(mod, L)  # plan exists: immutable
@@ -62,5 +65,30 @@ def test_autotune_ops_mask(oracle):
         h = rng.standard_normal(2 * 768).astype(np.float32)
         o = oracle.plan(OFO, 768)
         assert fp64.rel_rms(gpu_conv(f, x, h), o.conv(x, o.get_kernel(h))) <= 2e-6
+    finally:
+        fs.destroy()
+
+
+def test_autotune_keeps_the_tables_of_existing_plans(oracle):
+    # tuning one size must not free tables that a plan made before it uses
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    fs = fftset_amd.FFTSet()
+    try:
+        f = fs.create_fft(fftset_amd.FREQ_OFFSET_REAL, 4096)
+        before = fs.table_bytes
+        assert before > 0
+        fs.autotune(fftset_amd.FREQ_OFFSET_REAL, 4608, ops=(fftset_amd.OP_CONV,))
+        assert fs.table_bytes == before
+        rng = np.random.default_rng(4096)
+        x = rng.standard_normal(2 * 4096).astype(np.float32)
+        h = rng.standard_normal(2 * 4096).astype(np.float32)
+        o = oracle.plan(OFO, 4096)
+        assert fp64.rel_rms(gpu_conv(f, x, h), o.conv(x, o.get_kernel(h))) <= 2e-6
+        g = fs.create_fft(fftset_amd.FREQ_OFFSET_REAL, 4608)
+        o2 = oracle.plan(OFO, 4608)
+        x2 = rng.standard_normal(2 * 4608).astype(np.float32)
+        h2 = rng.standard_normal(2 * 4608).astype(np.float32)
+        assert fp64.rel_rms(gpu_conv(g, x2, h2), o2.conv(x2, o2.get_kernel(h2))) <= 2e-6
     finally:
         fs.destroy()
