@@ -167,6 +167,7 @@ def _check(what):
 SELECT_GENERIC = 1
 SELECT_PERSISTENT = 2
 SELECT_FOUR_STEP_FUSED = 4
+SELECT_FOUR_STEP_SPLIT = 8
 
 
 def kernel_select(flags):

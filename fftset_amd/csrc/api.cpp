@@ -15,6 +15,7 @@
 
 #include "context.h"
 #include "fs1.h"
+#include "fsr.h"
 #include "planner.h"
 
 static const fftset_modulation g_mod_fo = {fsa::MOD_FO};
@@ -309,6 +310,13 @@ int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A, int
         if (!p->fs.tfs_fo || !p->fs.trow_fo)
             return FFTSET_E_NOMEM;
     }
+    if (fsr_matches(p->fs.colc, p->fs.rowc)) {
+        // the CU-resident kernel's closing columns: first pass of [15, 16]
+        p->fs.tw_close = get_table(ctx, (long long)p->fs.A, 1, 14, 16, st);
+        if (!p->fs.tw_close)
+            return FFTSET_E_NOMEM;
+        p->fs.fsr = true;
+    }
     p->engine = ENG_FOUR_STEP;
     return FFTSET_OK;
 }
@@ -317,6 +325,44 @@ static int plan_four_step(fftset_fft* p, Ctx* ctx, hipStream_t st)
 {
     const TuneEntry* t = find_tune(ctx, p->mod, p->lfft);
     return plan_four_step_with(p, ctx, st, t ? t->A : 0, t && t->chain.empty() ? t->G : 0);
+}
+
+// True when (plan, op) runs the CU-resident kernel under the current
+// kernel selection.
+static bool use_fsr(const fftset_fft* p, int op)
+{
+    return p->engine == ENG_FOUR_STEP && p->fs.fsr && fsr_op(p->mod, op) && !generic_forced() &&
+           !persistent_mode() && !four_step_fused() && !four_step_split();
+}
+
+static int run_fsr(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
+                   long long in_st, const float* kern, long long kern_st, float* spill, long long spill_st,
+                   hipStream_t st, float* spec, long long spec_st)
+{
+    const FourStep& fs = p->fs;
+    FsrArgs a{};
+    a.tw_col = fs.colc.p[0].tw;
+    a.tw_close = fs.tw_close;
+    a.tw_row = fs.rowc.p[0].tw;
+    a.tfs = fs.tfs;
+    a.tfs_in = p->mod == MOD_FO ? fs.tfs_fo : fs.tfs;
+    a.trow = fs.trow_fo;
+    a.tmod = p->tmod;
+    a.ntrans = count;
+    a.conj_kern = op == OP_GRAD_IN;
+    a.in = in;
+    a.in_st = in_st;
+    a.out = out;
+    a.out_st = out_st;
+    a.kern = kern;
+    a.kern_st = kern_st;
+    a.spec = spec;
+    a.spec_st = spec_st;
+    a.spill = (float2*)spill;
+    a.spill_st = spill_st / 2;
+    const int opk = op == OP_FWD ? FSR_FWD : op == OP_INV ? FSR_INV : op == OP_KERN ? FSR_KERN : FSR_CONV;
+    count_launch(true);
+    return launch_fsr(p->mod, opk, a, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
 }
 
 static int run_four_step(const fftset_fft* p, int op, long long count, float* out, long long out_st,
@@ -378,6 +424,8 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
         const int opk = op == OP_FWD ? FS1_FWD : op == OP_INV ? FS1_INV : FS1_CONV;
         return ok(launch_fs1(p->mod, opk, f, fs1_grid(count), st)) ? FFTSET_OK : FFTSET_E_DEVICE;
     }
+    if (use_fsr(p, op))  // work (at least kFsrSpill complex values per transform) holds the spill rows
+        return run_fsr(p, op, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st, spec, spec_st);
     switch (op) {
     case OP_FWD:
         io(ca, in, in_st, work, work_st);
@@ -432,6 +480,13 @@ static int run_op(const fftset_fft* p, int op, long long count, float* out, long
         if (rc != FFTSET_OK)
             return rc;
         work_st = len2;
+    } else if (op == OP_KERN && !work && use_fsr(p, op)) {
+        // the resident kernel's spill rows (get_kernel has no work argument)
+        const long long sp = 2LL * kFsrSpill;
+        int rc = lease_scratch(p->ctx, st, (size_t)(count * sp), &lease, &work);
+        if (rc != FFTSET_OK)
+            return rc;
+        work_st = sp;
     }
     return run_four_step(p, op, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st, spec, spec_st);
 }
@@ -456,13 +511,15 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
     if (((uintptr_t)out | (uintptr_t)in | (uintptr_t)kern | (uintptr_t)work) & 7)
         return FFTSET_E_INVALID;
 
-    if (work && count > 1 && work_st < len2)
+    // get_kernel needs work only for the resident kernel's spill rows
+    const long long need_st = op == FFTSET_OP_GET_KERNEL ? 2LL * kFsrSpill : len2;
+    if (work && count > 1 && work_st < need_st)
         return FFTSET_E_INVALID;
     switch (op) {
     case FFTSET_OP_FORWARD: return run_op(p, OP_FWD, count, out, out_st, in, in_st, nullptr, 0, work, work_st, st);
     case FFTSET_OP_INVERSE: return run_op(p, OP_INV, count, out, out_st, in, in_st, nullptr, 0, work, work_st, st);
     case FFTSET_OP_GET_KERNEL:
-        return run_op(p, OP_KERN, count, out, out_st, in, in_st, nullptr, 0, nullptr, 0, st);
+        return run_op(p, OP_KERN, count, out, out_st, in, in_st, nullptr, 0, work, work_st, st);
     case FFTSET_OP_CONV:
         return run_op(p, OP_CONV, count, out, out_st, in, in_st, kern, kern_st, work, work_st, st);
     case FFTSET_OP_CONV_GRAD_INPUT:
@@ -496,7 +553,9 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
             }
             w2 = four ? s : nullptr;
         }
-        int rc = run_op(p, OP_KERN, count, w, wst, in, in_st, nullptr, 0, nullptr, 0, st);
+        // (a four-step get_kernel takes its spill rows from w2, which the
+        // second step uses only after it)
+        int rc = run_op(p, OP_KERN, count, w, wst, in, in_st, nullptr, 0, w2, len2, st);
         if (rc != FFTSET_OK)
             return rc;
         return run_op(p, OP_GRAD_IN, count, out, out_st, kern, kern_st, w, wst, w2, len2, st);

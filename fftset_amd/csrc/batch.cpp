@@ -26,6 +26,7 @@
 
 #include "context.h"
 #include "fs1.h"
+#include "fsr.h"
 #include "planner.h"
 #include "refpass.h"
 
@@ -60,6 +61,7 @@ struct BatchClass {
     std::vector<int> group_idx;    // this class's entries of group_ranges
     long long pass_off = 0;        // per-pass baseline: offset of its work tiles (float2)
     float* work_base = nullptr;    // four-step: its first work row in the pool (host view)
+    bool fsr_ok = false;           // a 240 x 256 four-step class the CU-resident kernel can run
 };
 
 // A contiguous run of tiles launched together with one workgroup size.
@@ -166,6 +168,7 @@ static int plan_class(fftset_batch* b, BatchClass& k, hipStream_t)
         return FFTSET_OK;
     }
     k.fs = p->fs;
+    k.fsr_ok = k.fs.fsr && !b->fs_tiles;  // four-step steps as ragged tiles keep the split launches
     if (k.fs.cols.E != 16 || k.fs.rows.E != 16)
         return FFTSET_E_UNSUPPORTED;
     r.A = (int)k.fs.A;
@@ -278,7 +281,7 @@ static int commit(fftset_batch* b, hipStream_t st)
             return rc;
         if (class_lds_bytes(k) > kMaxLdsBytes)
             return FFTSET_E_UNSUPPORTED;
-        if (k.four_step && k.op != OP_KERN)
+        if (k.four_step && (k.op != OP_KERN || k.fsr_ok))
             work_floats += 2ull * k.p->lfft * k.jobs.size();
         if (!k.internal)
             b->alg_bytes += op_bytes(k.op, k.p->lfft) * (long long)k.jobs.size();
@@ -429,7 +432,7 @@ static int commit(fftset_batch* b, hipStream_t st)
         const size_t nj = k.jobs.size();
         const size_t o_in = o_ptr[4 * ci], o_out = o_ptr[4 * ci + 1];
         const size_t o_k = o_ptr[4 * ci + 2], o_w = o_ptr[4 * ci + 3];
-        if (k.four_step && k.op != OP_KERN)
+        if (k.four_step && (k.op != OP_KERN || k.fsr_ok))
             b->classes[ci].work_base = wp;
         const void** hin = (const void**)(host.data() + o_in);
         void** hout = (void**)(host.data() + o_out);
@@ -441,7 +444,7 @@ static int commit(fftset_batch* b, hipStream_t st)
             hout[j] = jb.out;
             hk[j] = jb.kern;
             hw[j] = nullptr;
-            if (k.four_step && k.op != OP_KERN) {
+            if (k.four_step && (k.op != OP_KERN || k.fsr_ok)) {
                 hw[j] = wp;
                 wp += 2ull * k.p->lfft;
             }
@@ -499,6 +502,13 @@ static bool fs1_class(const BatchClass& k)
            fs1_matches(k.fs.colc, k.fs.rowc, k.fs.cols.T, k.fs.rows.T);
 }
 
+// True when the class runs the CU-resident kernel (one launch, step 0).
+static bool fsr_class(const BatchClass& k)
+{
+    return k.fsr_ok && fsr_op(k.rc.mod, k.op) && !generic_forced() && !persistent_mode() && !four_step_fused() &&
+           !four_step_split();
+}
+
 static int launch_fs_class(const BatchClass& k, int q, hipStream_t st, long long j0 = 0, long long n = -1)
 {
     RClass r = k.rc;
@@ -540,6 +550,29 @@ static int launch_fs_class(const BatchClass& k, int q, hipStream_t st, long long
         count_launch(true);
         const int opk = r.op == OP_FWD ? FS1_FWD : r.op == OP_INV ? FS1_INV : FS1_CONV;
         return launch_fs1(r.mod, opk, f, fs1_grid(a.ntrans), st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
+    }
+    if (fsr_class(k)) {
+        // the whole transform on one CU, all steps in one launch (step 0);
+        // its spill rows are the start of each transform's work row
+        if (q != 0)
+            return FFTSET_OK;
+        FsrArgs f{};
+        f.tw_col = fs.colc.p[0].tw;
+        f.tw_close = fs.tw_close;
+        f.tw_row = fs.rowc.p[0].tw;
+        f.tfs = fs.tfs;
+        f.tfs_in = fo ? fs.tfs_fo : fs.tfs;
+        f.trow = fs.trow_fo;
+        f.tmod = r.tmod;
+        f.ntrans = a.ntrans;
+        f.conj_kern = a.conj_kern;
+        f.in_p = r.in;
+        f.out_p = r.out;
+        f.kern_p = conv ? r.kern : nullptr;
+        f.spill_p = r.work;
+        count_launch(true);
+        const int opk = r.op == OP_FWD ? FSR_FWD : r.op == OP_INV ? FSR_INV : r.op == OP_KERN ? FSR_KERN : FSR_CONV;
+        return launch_fsr(r.mod, opk, f, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
     }
     const ShapeD& sh = q == 1 ? fs.rows : fs.cols;
     a.c = q == 1 ? fs.rowc : fs.colc;
@@ -687,7 +720,7 @@ static int launch_stage0(const fftset_batch* b, hipStream_t st, bool branches)
     if (!fsc.empty()) {
         if (int rc = launch_fs_class(*fsc[0], 0, st))
             return rc;
-        if (!fs1_class(*fsc[0])) {  // one workgroup per transform: no column step to wait for
+        if (!fs1_class(*fsc[0]) && !fsr_class(*fsc[0])) {  // one launch per transform: no column step to wait for
             if (hipEventRecord(b->mid, st) != hipSuccess)
                 return FFTSET_E_DEVICE;
             after = b->mid;

@@ -43,6 +43,11 @@ struct FourStep {
     // W_n^{n2 k1} W_{4n}^{n2} = W_{4n}^{n2 (4 k1 + 1)}
     const float2* tfs_fo = nullptr;
     const float2* trow_fo = nullptr;
+    // The 240 x 256 split with chains [16, 15] and [16, 16]: the CU-resident
+    // kernel (kernels_fsr_*.hip) runs it, with the closing columns' first
+    // pass table W_240^{u k} at (k-1)*16 + u (chain [15, 16]).
+    bool fsr = false;
+    const float2* tw_close = nullptr;
 };
 
 // Autotuned choices for one (modulation, size): radix chain and packing of

@@ -166,6 +166,7 @@ int persistent_waves();
 bool persistent_mode();
 bool generic_forced();
 bool four_step_fused();
+bool four_step_split();
 void count_launch(bool specialised);
 
 // out[r*cols + c] = W_N^{((a0 + r) * c) mod N}, computed in FP64.

@@ -7,6 +7,7 @@
 #include <mutex>
 
 #include "context.h"
+#include "fsr.h"
 
 namespace fsa {
 
@@ -46,7 +47,7 @@ static size_t work_per_transform(const fftset_fft* p, int op)
     const size_t len2 = 2ull * p->lfft;
     const bool four = p->engine == ENG_FOUR_STEP;
     switch (op) {
-    case FFTSET_OP_GET_KERNEL: return 0;
+    case FFTSET_OP_GET_KERNEL: return four && p->fs.fsr ? 2ull * kFsrSpill : 0;  // the resident kernel's spill
     case FFTSET_OP_CONV_GRAD_KERNEL: return len2 * (four ? 2 : 1);
     default: return four ? len2 : 0;
     }
@@ -85,10 +86,11 @@ extern "C" int fftset_fft_execute_noalloc(const struct fftset_fft* fft, int op, 
     // every operation that has one, a second region after it for a four-step
     // CONV_GRAD_KERNEL
     const long long len2 = 2LL * fft->lfft;
+    const long long wst = op == FFTSET_OP_GET_KERNEL ? (long long)per : len2;
     float* work = per ? w : nullptr;
     float* work2 = per ? w + (long long)count * len2 : nullptr;
     rc = execute(fft, op, (long long)count, out, (long long)out_stride, in, (long long)in_stride, kern,
-                 (long long)kern_stride, work, per ? len2 : 0, (hipStream_t)stream, work2);
+                 (long long)kern_stride, work, per ? wst : 0, (hipStream_t)stream, work2);
     set_error(rc);
     return rc;
 }

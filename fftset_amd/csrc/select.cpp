@@ -5,7 +5,8 @@
 // specialised kernel unless FFTSET_SELECT_GENERIC is set; FFTSET_SELECT_
 // PERSISTENT switches the LDS and four-step launches to resident grids with
 // register prefetch; FFTSET_SELECT_FOUR_STEP_FUSED runs 61440-bin transforms
-// with one workgroup per transform instead of column and row launches.
+// with one workgroup per transform and a scratch slot, FFTSET_SELECT_FOUR_
+// STEP_SPLIT with column and row launches, instead of the CU-resident kernel.
 // The environment (FFTSET_AMD_GENERIC, FFTSET_AMD_PERSIST) gives the initial
 // value, read once when the library loads; launches read an atomic, never
 // the environment.
@@ -41,6 +42,11 @@ bool four_step_fused()
     return (g_select.load(std::memory_order_relaxed) & FFTSET_SELECT_FOUR_STEP_FUSED) != 0;
 }
 
+bool four_step_split()
+{
+    return (g_select.load(std::memory_order_relaxed) & FFTSET_SELECT_FOUR_STEP_SPLIT) != 0;
+}
+
 void count_launch(bool specialised)
 {
     (specialised ? g_specialised : g_generic).fetch_add(1, std::memory_order_relaxed);
@@ -53,7 +59,8 @@ extern "C" {
 unsigned fftset_kernel_select(unsigned flags)
 {
     return fsa::g_select.exchange(flags &
-                                  (FFTSET_SELECT_GENERIC | FFTSET_SELECT_PERSISTENT | FFTSET_SELECT_FOUR_STEP_FUSED));
+                                  (FFTSET_SELECT_GENERIC | FFTSET_SELECT_PERSISTENT | FFTSET_SELECT_FOUR_STEP_FUSED |
+                                   FFTSET_SELECT_FOUR_STEP_SPLIT));
 }
 
 void fftset_kernel_counts(unsigned long long* specialised, unsigned long long* generic)

@@ -142,7 +142,9 @@ int fftset_last_error(void);
  *                     spectrum as kern, instead of CONV_GRAD_KERNEL's
  *                     get_kernel of the signal plus that pass.
  * work (may be NULL) provides count*work_stride floats of scratch for the
- * operations that need it; when NULL the library uses its own, one buffer
+ * operations that need it (work_stride >= 2*complex_bins; GET_KERNEL needs
+ * only 16384 floats per transform, for 61440-bin plans); when NULL the
+ * library uses its own, one buffer
  * per stream (so calls on different streams never share it). A buffer that
  * must grow is replaced, and the old one is freed only by fftset_destroy,
  * so work=NULL calls on one stream from several host threads are safe;
@@ -162,7 +164,8 @@ int fftset_fft_execute(const struct fftset_fft *fft, int op, size_t count, float
  * (64 MiB, held by each GPU the library runs on until the process ends).
  * Needs (floats): four-step plans (complex_bins > 16384), every operation
  * but GET_KERNEL: count * 2 * complex_bins, CONV_GRAD_KERNEL twice that;
- * LDS plans: CONV_GRAD_KERNEL count * 2 * complex_bins, others none.
+ * GET_KERNEL of a 61440-bin plan: count * 16384; LDS plans:
+ * CONV_GRAD_KERNEL count * 2 * complex_bins, others none.
  * Larger calls fail with FFTSET_E_TOO_LARGE before any work is queued.
  * Every call on one GPU shares that GPU's array: calls that overlap in time
  * on one GPU (different streams or host threads without ordering) race and
@@ -315,15 +318,20 @@ void fftset_blockconv_destroy(struct fftset_blockconv *bc);
 /* Kernel selection, process-wide (for tests and comparisons). Chains with a
  * compile-time specialisation run it by default; FFTSET_SELECT_GENERIC runs
  * the runtime-chain kernels instead, FFTSET_SELECT_PERSISTENT resident-size
- * grids with register prefetch. FFTSET_SELECT_FOUR_STEP_FUSED runs 61440-bin
- * transforms with one workgroup per transform (intermediate in a
- * per-workgroup scratch slot) instead of separate column and row launches
- * (measured slower on MI355X; kept for comparison). Initial value from the
- * environment (FFTSET_AMD_GENERIC=1, FFTSET_AMD_PERSIST=1), read once at
- * load. Returns the previous flags. */
+ * grids with register prefetch. 61440-bin transforms (the 240 x 256 split)
+ * run by default with the whole transform resident on one CU (one launch,
+ * intermediate in LDS and VGPRs), except FREQOFFSETREAL forward and inverse;
+ * FFTSET_SELECT_FOUR_STEP_SPLIT runs them with separate column and row
+ * launches (intermediate in the work buffer), and
+ * FFTSET_SELECT_FOUR_STEP_FUSED with one workgroup per transform and its
+ * intermediate in a per-workgroup scratch slot (measured slower on MI355X;
+ * kept for comparison). Kernel spectra are interchangeable between the
+ * three. Initial value from the environment (FFTSET_AMD_GENERIC=1,
+ * FFTSET_AMD_PERSIST=1), read once at load. Returns the previous flags. */
 #define FFTSET_SELECT_GENERIC          1u
 #define FFTSET_SELECT_PERSISTENT       2u
 #define FFTSET_SELECT_FOUR_STEP_FUSED  4u
+#define FFTSET_SELECT_FOUR_STEP_SPLIT  8u
 unsigned fftset_kernel_select(unsigned flags);
 /* Launches of specialised and of runtime-chain kernels issued by the
  * drop-in and uniform-batch calls since the previous call (which resets

@@ -420,8 +420,8 @@ def test_specialised_kernels_match_generic(tuned_fs, mod, L, op):
                                     (CX, "conv"), (FO, "grad")])
 def test_one_workgroup_four_step_equals_split_launches(fs, mod, op):
     # 61440 bins: the opt-in one-workgroup-per-transform kernel (intermediate
-    # in a per-workgroup scratch slot) against the default separate column and
-    # row launches (intermediate in the work buffer): the same passes,
+    # in a per-workgroup scratch slot) against separate column and row
+    # launches (intermediate in the work buffer): the same passes,
     # twiddles and order, so bit-identical; more transforms than workgroups,
     # so slots are reused
     import torch
@@ -441,7 +441,11 @@ def test_one_workgroup_four_step_equals_split_launches(fs, mod, op):
         torch.cuda.synchronize()
         return y.cpu().numpy()
 
-    split = run()
+    prev = fftset_amd.kernel_select(fftset_amd.SELECT_FOUR_STEP_SPLIT)
+    try:
+        split = run()
+    finally:
+        fftset_amd.kernel_select(prev)
     prev = fftset_amd.kernel_select(fftset_amd.SELECT_FOUR_STEP_FUSED)
     try:
         fused = run()
