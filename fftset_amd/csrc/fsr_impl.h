@@ -199,14 +199,17 @@ __device__ __forceinline__ void col_load(float2* P, const float* in, int c, int 
 // (e[0] may carry a modulation factor, as tfs_fo's rows do).
 struct Tw7 {
     float2 e[7];  // W^{x b}; W^{16 x b'}, b' = 1..3; W^{64 x a}, a = 1..3
+    // e[b'] <- e[0] e[b']: then every twiddle is at most one product away
+    __device__ __forceinline__ void prep()
+    {
+#pragma unroll
+        for (int b = 1; b < 4; ++b) e[b] = cmul(e[0], e[b]);
+    }
+    // v times the twiddle of row (element) k, after prep()
     __device__ __forceinline__ float2 apply(float2 v, int k) const
     {
         const int b = k & 3, a = k >> 2;
-        float2 w = e[0];
-        if (b)
-            w = cmul(w, e[b]);
-        if (a)
-            w = cmul(w, e[3 + a]);
+        const float2 w = a ? cmul(e[b], e[3 + a]) : e[b];
         return cmul(v, w);
     }
 };
@@ -286,6 +289,7 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
 #pragma unroll
             for (int q = 0; q < 15; ++q) v[q] = tile[(15 * lt + q) * 32 + g];
             DFT<15>::run(v);
+            e.prep();
 #pragma unroll
             for (int k = 0; k < 15; ++k) {
                 const float2 w = e.apply(v[k], k);
@@ -413,7 +417,7 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
             for (int i = 0; i < rn; ++i) {
                 const int r = rr + rrow * i, j = rj;
                 const int lo2 = kB * r + j;
-                Pow16 tw;
+                Pow16 tw;  // W_256^{j k}, as in the first pass
                 tw.load(fresh(a.tw_row), 16, j);
                 Tw7 e;  // W_n^{(j + 16 k) k1}, k1 = 80 q + r
                 e.e[0] = *ub(tf, 0, lo2);
@@ -428,6 +432,7 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
 #pragma unroll
                 for (int s = 1; s < 16; ++s) v[s] = tw.apply(v[s], s);
                 DFT<16>::run(v);
+                e.prep();
                 const int pw = S::pa(fresh_int(r), fresh_int(j));
 #pragma unroll
                 for (int k = 0; k < 16; ++k) lds[S::at_a(pw, k)] = e.apply(v[k], k);
