@@ -1,6 +1,6 @@
 // fsr.h -- 61440-bin transforms (240 x 256) with the whole transform
-// resident on one CU (kernels_fsr.hip): declarations shared with the host
-// runtime.
+// resident on one CU (fsr_impl.h, instantiated in kernels_fsr_cx.hip and
+// kernels_fsr_fo.hip): declarations shared with the host runtime.
 #pragma once
 
 #include "plan_desc.h"
