@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="ragged", choices=["ragged", "c2", "c3", "c5", "large", "block"])
     ap.add_argument("--transforms", type=int, default=20000, help="ragged: batch size")
+    ap.add_argument("--align", type=int, default=0, metavar="BYTES",
+                    help="ragged: start every transform's buffers at a multiple of BYTES (0: back to back, "
+                         "so starts are only 64 B aligned)")
     ap.add_argument("--threads", type=int, default=256, choices=[256, 512], help="ragged: lanes per workgroup")
     ap.add_argument("--no-graph", action="store_true", help="ragged: launch phases directly")
     ap.add_argument("--persistent", action="store_true",
@@ -62,8 +65,11 @@ def parse():
     ap.add_argument("--baselines", default="group,pass,reference,transform",
                     help="ragged: comma list of baselines to time (group, pass, reference, transform, none)")
     ap.add_argument("--four-step-fused", action="store_true",
-                    help="run 61440-bin transforms with one workgroup per transform (intermediate in a per-workgroup "
-                         "scratch slot) instead of separate column and row launches")
+                    help="run 61440-bin transforms with the 1024-lane one-workgroup-per-transform kernel (intermediate "
+                         "in a per-workgroup scratch slot) instead of the CU-resident kernel")
+    ap.add_argument("--four-step-split", action="store_true",
+                    help="run 61440-bin transforms as separate column and row launches instead of the CU-resident "
+                         "kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-check", action="store_true", help="skip the parity spot check")
@@ -340,7 +346,9 @@ def workload_ragged(torch, fs, rng_np, args):
     rng = np.random.default_rng(4)
     sizes = np.array(RAGGED_SIZES)[rng.integers(0, len(RAGGED_SIZES), n)]
     is_conv = rng.random(n) < 0.5
-    off = np.concatenate([[0], np.cumsum(2 * sizes)])
+    # transform i at float offset off[i]: back to back, or each start rounded up to --align bytes
+    pad = max(args.align // 4, 1)
+    off = np.concatenate([[0], np.cumsum((2 * sizes + pad - 1) // pad * pad)])
     total = int(off[-1])
     g = torch.Generator(device="cuda").manual_seed(4)
     x = torch.randn(total, device="cuda", generator=g)
@@ -377,7 +385,7 @@ def workload_ragged(torch, fs, rng_np, args):
     kernels = {"ragged_batch": lambda: b.launch()}
     bytes_per = {"ragged_batch": b.bytes}
     cfg = {"workload": "ragged batch: COMPLEX forward + FREQOFFSETREAL conv, one graph launch",
-           "transforms": n, "sizes": RAGGED_SIZES, "conv_fraction": round(float(is_conv.mean()), 4),
+           "transforms": n, "sizes": RAGGED_SIZES, "align_bytes": args.align, "conv_fraction": round(float(is_conv.mean()), 4),
            "algorithmic_bytes": b.bytes, "tiles": info["tiles"], "classes": info["classes"],
            "four_step_classes": info["four_step_classes"], "lds_bytes": info["lds_bytes"],
            "threads": args.threads, "graph": not args.no_graph, "schedule": args.sched,
@@ -398,7 +406,7 @@ def workload_ragged(torch, fs, rng_np, args):
                 idx = np.nonzero((sizes == L) & (is_conv == conv))[0][:4]
                 key = f"{'conv' if conv else 'fwd'}{L}"
                 for i in idx:
-                    a, e = int(off[i]), int(off[i + 1])
+                    a, e = int(off[i]), int(off[i]) + 2 * int(sizes[i])
                     xs = x[a:e].cpu().numpy()
                     ys = y[a:e].cpu().numpy()
                     if conv:
@@ -483,10 +491,10 @@ def cpu_baseline_ragged(x_dev, sizes, is_conv, off, h_dev, seconds):
                     idx = [j for j in sel if sizes[j] == L and is_conv[j] == conv]
                     if not idx:
                         continue
-                    xs = np.stack([x_dev[int(off[j]):int(off[j + 1])].cpu().numpy() for j in idx])
+                    xs = np.stack([x_dev[int(off[j]):int(off[j]) + 2 * int(sizes[j])].cpu().numpy() for j in idx])
                     if conv:
                         o = orc.plan(MOD_FREQ_OFFSET_REAL, L)
-                        hs = np.stack([h_dev[int(off[j]):int(off[j + 1])].cpu().numpy() for j in idx])
+                        hs = np.stack([h_dev[int(off[j]):int(off[j]) + 2 * int(sizes[j])].cpu().numpy() for j in idx])
                         work.append((o, OP_CONV, xs, o.get_kernel(hs), 24 * L * len(idx)))
                     else:
                         work.append((orc.plan(MOD_COMPLEX, L), OP_FORWARD, xs, None, 16 * L * len(idx)))
@@ -566,8 +574,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.four_step_fused:
-        fftset_amd.kernel_select(fftset_amd.SELECT_FOUR_STEP_FUSED)
+    if args.four_step_fused or args.four_step_split:
+        fftset_amd.kernel_select((fftset_amd.SELECT_FOUR_STEP_FUSED if args.four_step_fused else 0) |
+                                 (fftset_amd.SELECT_FOUR_STEP_SPLIT if args.four_step_split else 0))
     tune = args.tune_file if args.tune_file and os.path.exists(args.tune_file) else None
     fs = fftset_amd.FFTSet(tune_file=tune)
     rng = np.random.default_rng(0)
@@ -575,6 +584,9 @@ def main():
              "large": workload_large, "block": workload_block}[args.workload]
     built = build(torch, fs, rng, args)
     kernels, bytes_per, cfg, check, cpu_sample = built[:5]
+    if args.workload in ("ragged", "large"):
+        cfg["four_step_kernel"] = ("one_workgroup_fused" if args.four_step_fused else
+                                   "column_row_launches" if args.four_step_split else "cu_resident")
     baselines = built[5] if len(built) > 5 else None
     timer = Timer(torch)
 

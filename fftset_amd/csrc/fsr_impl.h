@@ -14,7 +14,9 @@
 //   k >= 5 (160 rows):                    VGPRs, 80 complex values per lane
 //                                          (lane (g, b) of a 32-column chunk).
 // Columns: 8 chunks of 32 adjacent columns (chain [16, 15]) in a 60 KB LDS
-//   tile over the LDS of rows 48..79; global loads in 256 B runs; outputs
+//   tile over the LDS of rows 48..79; global loads in 256 B (COMPLEX) or
+//   128 B (FREQOFFSETREAL) runs that start on a 128-byte line whatever the
+//   input's alignment (the chunks start at the matching column); outputs
 //   times W_n^{n2 k1} to their place above.
 // Rows: 3 batches of 80 rows in LDS (chain [16, 16], in place; the spilled
 //   rows return first):
@@ -101,17 +103,26 @@ __device__ __forceinline__ int fresh_int(int x)
     return x;
 }
 
-// Column lanes: column g of a 32-column chunk, lane lt (block b = lt), and
-// their offset lo = 256 lt + g into a 3840-element stripe.
+// Column lanes: lane lt (block b = lt) of a 32-column chunk. The chunks
+// start at column s (0 <= s < 32) of the transform, chosen so that every
+// chunk's run of 32 columns in a row begins on a 128-byte line of the
+// input: a run that straddled two lines would have each boundary line
+// fetched once for either chunk (the loads are nontemporal). In chunk c the
+// lane holds column 32 cc(c) + g, with g = (lane + s) & 31 and d the carry.
 struct ColLane {
-    int g, lt, lo;
-    __device__ __forceinline__ ColLane()
+    int g, lt, d, lo;  // lo = 256 lt + g
+    __device__ __forceinline__ explicit ColLane(int s)
     {
         const int t = fresh_tid();
-        g = t & 31;
+        const int x = (t & 31) + s;
+        g = x & 31;
+        d = x >> 5;
         lt = t >> 5;
         lo = kB * lt + g;
     }
+    __device__ __forceinline__ int cc(int c) const { return (c + d) & 7; }
+    // 256 lt + the lane's column in chunk c
+    __device__ __forceinline__ int at(int c) const { return lo + 32 * cc(c); }
 };
 
 // LDS position of element p of batch row r, without padding (a batch of 80
@@ -124,7 +135,8 @@ struct ColLane {
 // an access costs one VALU instruction at most:
 //   pa(r, j) ^ at_a(s): element 16 s + j   (radix-16 pass over stride 16)
 //   pb(r, b) ^ s:       element 16 b + s   (radix-16 pass over stride 1)
-//   pc(r, g) . at_c(c): element 32 c + g   (column chunk c, column g)
+//   pc(r, g) . at_c(c): element 32 c + g   (column chunk c, column g; c may
+//                       be a lane value)
 template <bool ACROSS>
 struct Swz;
 
@@ -176,18 +188,18 @@ struct Pow16 {
     }
 };
 
-// Column inputs of chunk c for pass-1 lane u: elements n1 = u + 15 q of
-// column n2, at 3840 q + 32 c + lo. COMPLEX: the value (conjugated for the
+// Column inputs for pass-1 lane u: elements n1 = u + 15 q of column n2, at
+// 3840 q + lo (lo = 256 u + n2). COMPLEX: the value (conjugated for the
 // inverse). FREQOFFSETREAL: the real pair (x[e], x[n + e]).
 template <int MOD, int OPK>
-__device__ __forceinline__ void col_load(float2* P, const float* in, int c, int lo)
+__device__ __forceinline__ void col_load(float2* P, const float* in, int lo)
 {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         if constexpr (MOD == MOD_FO) {
-            P[q] = make_float2(ldg(ub(in, 3840 * q + 32 * c, lo)), ldg(ub(in, kN + 3840 * q + 32 * c, lo)));
+            P[q] = make_float2(ldg(ub(in, 3840 * q, lo)), ldg(ub(in, kN + 3840 * q, lo)));
         } else {
-            const float2 v = ldg(ub((const float2*)in, 3840 * q + 32 * c, lo));
+            const float2 v = ldg(ub((const float2*)in, 3840 * q, lo));
             P[q] = OPK == FSR_INV ? cconj(v) : v;
         }
     }
@@ -228,6 +240,9 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
     const float* in = a.in_p ? a.in_p[t] : a.in + t * a.in_st;
     float* out = a.out_p ? a.out_p[t] : a.out + t * a.out_st;
     float2* spill = a.spill_p ? (float2*)a.spill_p[t] : a.spill + t * a.spill_st;
+    // first column of the chunks (ColLane): in + s elements is 128-byte aligned
+    const unsigned long long neg = 0ull - reinterpret_cast<unsigned long long>(in);
+    const int s = MOD == MOD_FO ? (int)((neg >> 2) & 31) : (int)((neg >> 3) & 15);
 
     // R[c][k - 5]: row b + 16 k of column 32 c + g (k = 5..14) after the
     // columns; for a convolution, after the row batches R[c][k] holds the
@@ -241,7 +256,7 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
     // into the output twiddles.
     Pow16 tw1;
     {
-        const ColLane cl;
+        const ColLane cl(s);
         const int u = cl.lt < 15 ? cl.lt : 14;
         tw1.load(fresh(a.tw_col), 15, u);
         if constexpr (MOD == MOD_FO) {
@@ -253,21 +268,21 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
     }
     float2 P[16];  // this chunk's pass-1 inputs (loaded one chunk ahead)
     {
-        const ColLane cl;
+        const ColLane cl(s);
         if (cl.lt < 15)
-            col_load<MOD, OPK>(P, in, 0, cl.lo);
+            col_load<MOD, OPK>(P, in, cl.at(0));
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        const ColLane cl;
-        const int g = cl.g, lt = cl.lt, lo = cl.lo;
+        const ColLane cl(s);
+        const int g = cl.g, lt = cl.lt, cc = cl.cc(c), lo = cl.lo + 32 * cc, n2 = 32 * cc + g;
         // pass-2 twiddles of this chunk, issued before the next chunk's loads
         Tw7 e;
-        e.e[0] = *ub(a.tfs_in, 32 * c, lo);
+        e.e[0] = *ub(a.tfs_in, 0, lo);
 #pragma unroll
-        for (int b = 1; b < 4; ++b) e.e[b] = *ub(a.tfs, 4096 * b + 32 * c, g);
+        for (int b = 1; b < 4; ++b) e.e[b] = *ub(a.tfs, 4096 * b, n2);
 #pragma unroll
-        for (int m = 1; m < 4; ++m) e.e[3 + m] = *ub(a.tfs, 16384 * m + 32 * c, g);
+        for (int m = 1; m < 4; ++m) e.e[3 + m] = *ub(a.tfs, 16384 * m, n2);
         if (lt < 15) {  // first pass, radix 16 over n1 = lt + 15 q
             float2 v[16];
             static_for<16>([&](auto Q) {
@@ -281,7 +296,7 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
 #pragma unroll
             for (int k = 0; k < 16; ++k) tile[(lt + 15 * k) * 32 + g] = tw1.apply(v[k], k);
             if (c < 7)
-                col_load<MOD, OPK>(P, in, c + 1, lo);  // in flight over pass 2
+                col_load<MOD, OPK>(P, in, cl.at(c + 1));  // in flight over pass 2
         }
         __syncthreads();
         {  // second pass, radix 15: outputs k1 = lt + 16 k
@@ -294,9 +309,9 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
             for (int k = 0; k < 15; ++k) {
                 const float2 w = e.apply(v[k], k);
                 if (k < 3)
-                    lds[S::at_c(S::pc(lt + 16 * k, g), c)] = w;
+                    lds[S::at_c(S::pc(lt + 16 * k, g), cc)] = w;
                 else if (k < 5)
-                    *ub(spill, 4096 * (k - 3) + 32 * c, lo) = w;
+                    *ub(spill, 4096 * (k - 3), lo) = w;
                 else {
                     R[c][k - 5] = w;
                     pin(R[c][k - 5]);
@@ -327,9 +342,10 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
         if (q > 0 && !kConv) {  // this batch's rows from VGPRs
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const ColLane cl;
+                const ColLane cl(s);
 #pragma unroll
-                for (int kk = 0; kk < 5; ++kk) lds[S::at_c(S::pc(cl.lt + 16 * kk, cl.g), c)] = R[c][5 * (q - 1) + kk];
+                for (int kk = 0; kk < 5; ++kk)
+                    lds[S::at_c(S::pc(cl.lt + 16 * kk, cl.g), cl.cc(c))] = R[c][5 * (q - 1) + kk];
             }
             __syncthreads();
         }
@@ -443,10 +459,10 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
                 // positions (row b + 16 kk of its columns)
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
-                    const ColLane cl;
+                    const ColLane cl(s);
 #pragma unroll
                     for (int kk = 0; kk < 5; ++kk) {
-                        const int pc = S::at_c(S::pc(cl.lt + 16 * kk, cl.g), c);
+                        const int pc = S::at_c(S::pc(cl.lt + 16 * kk, cl.g), cl.cc(c));
                         const float2 o = lds[pc];
                         lds[pc] = R[c][5 * q + kk];
                         R[c][5 * q + kk] = o;
@@ -472,29 +488,29 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
         Pow16 tw3;
         float2 sp[2];  // this chunk's two spilled rows (loaded one chunk ahead)
         {
-            const ColLane cl;
+            const ColLane cl(s);
             tw3.load(fresh(a.tw_close), 16, cl.lt);
 #pragma unroll
-            for (int k = 0; k < 2; ++k) sp[k] = *ub(spill, 4096 * k, cl.lo);
+            for (int k = 0; k < 2; ++k) sp[k] = *ub(spill, 4096 * k, cl.at(0));
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const ColLane cl;
-            const int g = cl.g, lt = cl.lt, lo = cl.lo;
+            const ColLane cl(s);
+            const int g = cl.g, lt = cl.lt, cc = cl.cc(c), lo = cl.lo + 32 * cc;
             float2 dm = make_float2(1.f, 0.f);  // FREQOFFSETREAL: W_{4n}^{256 lt + n2}
             if constexpr (MOD == MOD_FO)
-                dm = *ub(a.tmod, 32 * c, lo);
+                dm = *ub(a.tmod, 0, lo);
             {  // first pass of [15, 16], radix 15 over k1 = lt + 16 k
                 float2 v[15];
 #pragma unroll
                 for (int k = 0; k < 10; ++k) v[k] = R[c][k];
 #pragma unroll
-                for (int k = 10; k < 13; ++k) v[k] = lds[S::at_c(S::pc(lt + 16 * (k - 10), g), c)];
+                for (int k = 10; k < 13; ++k) v[k] = lds[S::at_c(S::pc(lt + 16 * (k - 10), g), cc)];
                 v[13] = sp[0];
                 v[14] = sp[1];
                 if (c < 7) {
 #pragma unroll
-                    for (int k = 0; k < 2; ++k) sp[k] = *ub(spill, 4096 * k + 32 * (c + 1), lo);
+                    for (int k = 0; k < 2; ++k) sp[k] = *ub(spill, 4096 * k, cl.at(c + 1));
                 }
                 DFT<15>::run(v);
 #pragma unroll
@@ -512,12 +528,12 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
                     static_for<16>([&](auto Kk) {
                         constexpr int k = decltype(Kk)::value;
                         const float2 w = rootmul<k, 64>(cmul(v[k], dm));
-                        stg(ub(out, 3840 * k + 32 * c, lo), w.x);
-                        stg(ub(out, kN + 3840 * k + 32 * c, lo), w.y);
+                        stg(ub(out, 3840 * k, lo), w.x);
+                        stg(ub(out, kN + 3840 * k, lo), w.y);
                     });
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) stg(ub((float2*)out, 3840 * k + 32 * c, lo), cconj(v[k]));
+                    for (int k = 0; k < 16; ++k) stg(ub((float2*)out, 3840 * k, lo), cconj(v[k]));
                 }
             }
             __syncthreads();

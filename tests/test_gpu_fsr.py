@@ -114,6 +114,57 @@ def test_spectra_interchange_with_split_launches(fs, mod):
         assert np.max(fp64.rel_rms(y.cpu().numpy(), ref)) <= fp64_tol(L)
 
 
+def _at(n, off, seed, scale=None):
+    # n transforms back to back starting `off` floats into a fresh buffer
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(n * 2 * L + 64)
+    if scale is not None:
+        v[off:off + n * 2 * L] *= np.tile(scale, n)
+    return torch.from_numpy(v.astype(np.float32)).cuda()[off:off + n * 2 * L].view(n, 2 * L)
+
+
+@pytest.mark.parametrize("mod", [CX, FO])
+@pytest.mark.parametrize("off_in,off_out", [(2, 2), (16, 16), (30, 30), (16, 0), (0, 26)])
+def test_resident_misaligned_buffers(fs, mod, off_in, off_out):
+    # buffers that start off a 128-byte line (the column chunks then start
+    # at another column): every operation against the two-launch path and
+    # FP64, with input and output offsets equal or different
+    n = 3
+    f = fs.create_fft(mod, L)
+    decay = np.exp(-np.arange(2 * L) / 4096.0)
+    x = _at(n, off_in, 70 + off_in)
+    h = _at(n, off_in, 71 + off_in, decay)
+    k = _at(n, off_in, 72)
+    f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
+    torch.cuda.synchronize()
+    ks = _split(lambda: _run_into(f, "kern", _at(n, off_out, 73), h))
+    assert np.max(fp64.rel_rms(k.cpu().numpy(), ks.cpu().numpy())) <= 1e-6
+    xs, hs = x.cpu().numpy(), h.cpu().numpy()
+    for op in ("fwd", "inv", "conv", "grad"):
+        kk = k if op in ("conv", "grad") else None
+        y = _run_into(f, op, _at(n, off_out, 74), x, kk)
+        ys = _split(lambda: _run_into(f, op, _at(n, off_out, 75), x, kk))
+        r = y.cpu().numpy()
+        assert np.all(np.isfinite(r)), op
+        assert np.max(fp64.rel_rms(r, ys.cpu().numpy())) <= 1e-6, op
+        if op == "fwd":
+            ref = fp64.cx_forward64(xs) if mod == CX else fp64.fo_forward64(xs)
+        elif op == "inv":
+            ref = fp64.cx_inverse64(xs) if mod == CX else fp64.fo_inverse64(xs)
+        elif op == "conv":
+            ref = fp64.fo_conv64(xs, hs) if mod == FO else fp64.cx_conv64(xs, hs)
+        else:
+            continue
+        assert np.max(fp64.rel_rms(r, ref)) <= fp64_tol(L), op
+
+
+def _run_into(f, op, y, x, k=None):
+    y.fill_(float("nan"))
+    f.execute(OPS[op], x.shape[0], y, x, kern=k)
+    torch.cuda.synchronize()
+    return y
+
+
 @pytest.mark.parametrize("mod,op", [(CX, "fwd"), (FO, "conv"), (CX, "kern")])
 def test_resident_in_place(fs, mod, op):
     # out == in (the reference allows aliasing): the columns read the input
