@@ -18,8 +18,9 @@ constexpr int kRaggedE = 16;
 constexpr int kIdleLane = 1 << 28;
 
 // FOLD_ST: the FREQOFFSETREAL demodulation folded into the convolution's
-// closing pass too (the 1024-lane kernel keeps it in the store: folding both
-// sides there exceeds 128 VGPRs).
+// closing pass too. Only the 256-lane kernel does: folding both sides takes
+// 132 VGPRs, above the 128 that let two 512-lane (or one 1024-lane)
+// workgroups share a CU.
 template <int MOD, int OP, bool FOLD_ST>
 __device__ __forceinline__ void ragged_lds(const RClass& k, const RTile& t, float2* smem)
 {
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
             if (ti >= a.ntiles)
                 return;  // the same value in every lane: the whole workgroup leaves
         }
-        ragged_tile<FS, (THREADS < 1024)>(a, ti, smem);
+        ragged_tile<FS, (THREADS < 512)>(a, ti, smem);
         if (!a.next)
             return;
         __syncthreads();  // LDS and s_tile are reused by the next tile
