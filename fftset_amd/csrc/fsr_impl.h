@@ -367,11 +367,14 @@ __global__ void __launch_bounds__(kThreads) k_fsr(const FsrArgs a)
         __syncthreads();
         if constexpr (kAcross) {
             // second pass, radix 16 over elements 16 b + s: outputs
-            // k2 = b + 16 k of row k1 = 80 q + r, stored transposed
+            // k2 = b + 16 k of row k1 = 80 q + r, stored transposed. The
+            // lanes' rows are rotated by s2 so that each run of 32 rows
+            // starts on a 128-byte line of the output.
+            const int s2 = (int)((0ull - reinterpret_cast<unsigned long long>(out)) >> 3) & 15;
             float2* o = (float2*)out + 80 * q;
 #pragma unroll 1
             for (int i = 0; i < rn; ++i) {
-                const int r = rr + rrow * i, b = rj;
+                const int r0 = rr + rrow * i + s2, r = r0 >= 80 ? r0 - 80 : r0, b = rj;
                 const int pb = S::pb(r, b);
                 float2 v[16];
 #pragma unroll
