@@ -16,6 +16,7 @@
 #include "context.h"
 #include "fs1.h"
 #include "fsr.h"
+#include "rcol.h"
 #include "planner.h"
 
 static const fftset_modulation g_mod_fo = {fsa::MOD_FO};
@@ -317,6 +318,7 @@ int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A, int
             return FFTSET_E_NOMEM;
         p->fs.fsr = true;
     }
+    p->fs.rcol = rcol_matches(p->fs.colc, p->fs.rowc);
     p->engine = ENG_FOUR_STEP;
     return FFTSET_OK;
 }
@@ -333,6 +335,41 @@ static bool use_fsr(const fftset_fft* p, int op)
 {
     return p->engine == ENG_FOUR_STEP && p->fs.fsr && fsr_op(p->mod, op) && !generic_forced() &&
            !persistent_mode() && !four_step_fused() && !four_step_split();
+}
+
+// True when (plan, op) runs the column-register kernel under the current
+// kernel selection (every operation of such a plan does).
+static bool use_rcol(const fftset_fft* p, int /*op*/)
+{
+    return p->engine == ENG_FOUR_STEP && p->fs.rcol && !generic_forced() && !persistent_mode() &&
+           !four_step_fused() && !four_step_split();
+}
+
+bool four_step_work(const fftset_fft* p, int op) { return p->engine == ENG_FOUR_STEP && !use_rcol(p, op); }
+
+static int run_rcol(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
+                    long long in_st, const float* kern, long long kern_st, hipStream_t st, float* spec,
+                    long long spec_st)
+{
+    const FourStep& fs = p->fs;
+    RcolArgs a{};
+    a.tw_row = fs.rowc.p[0].tw;
+    a.tfs = fs.tfs;
+    a.tfs_in = p->mod == MOD_FO ? fs.tfs_fo : fs.tfs;
+    a.tmod = p->tmod;
+    a.ntrans = count;
+    a.conj_kern = op == OP_GRAD_IN;
+    a.in = in;
+    a.in_st = in_st;
+    a.out = out;
+    a.out_st = out_st;
+    a.kern = kern;
+    a.kern_st = kern_st;
+    a.spec = spec;
+    a.spec_st = spec_st;
+    const int opk = op == OP_FWD ? RC_FWD : op == OP_INV ? RC_INV : op == OP_KERN ? RC_KERN : RC_CONV;
+    count_launch(true);
+    return launch_rcol((int)fs.A, p->mod, opk, a, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
 }
 
 static int run_fsr(const fftset_fft* p, int op, long long count, float* out, long long out_st, const float* in,
@@ -400,6 +437,8 @@ static int run_four_step(const fftset_fft* p, int op, long long count, float* ou
         a.out_st = ost;
     };
     auto ok = [](hipError_t e) { return e == hipSuccess; };
+    if (use_rcol(p, op))  // no work rows: the intermediate stays on the CU
+        return run_rcol(p, op, count, out, out_st, in, in_st, kern, kern_st, st, spec, spec_st);
     if (op != OP_KERN && !spec && four_step_fused() && !persistent_mode() && !generic_forced() &&
         fs1_matches(fs.colc, fs.rowc, fs.cols.T, fs.rows.T)) {
         // one workgroup per transform; work holds its scratch slots (at most
@@ -475,7 +514,7 @@ static int run_op(const fftset_fft* p, int op, long long count, float* out, long
         return run_lds(p, op, count, out, out_st, in, in_st, kern, kern_st, st, spec, spec_st);
     const long long len2 = 2LL * p->lfft;
     std::unique_lock<std::mutex> lease;
-    if (op != OP_KERN && !work) {
+    if (op != OP_KERN && !work && four_step_work(p, op)) {
         int rc = lease_scratch(p->ctx, st, (size_t)(count * len2), &lease, &work);
         if (rc != FFTSET_OK)
             return rc;
@@ -535,7 +574,7 @@ int execute(const fftset_fft* p, int op, long long count, float* out, long long 
         // spectrum of the signal (in) into a scratch region w, then the
         // signal-adjoint pass of the output gradient (kern) against it.
         // Four-step plans need a second region for their intermediate.
-        const bool four = p->engine == ENG_FOUR_STEP;
+        const bool four = four_step_work(p, OP_GRAD_IN);
         float* w = work;
         long long wst = work_st;
         float* w2 = four ? work2 : nullptr;

@@ -24,6 +24,7 @@
 
 #include "context.h"
 #include "planner.h"
+#include "rcol.h"
 
 namespace fsa {
 
@@ -114,7 +115,7 @@ static int score(Ctx* ctx, int mod, unsigned L, const Cand& c, Buffers& b, long 
     int rc = c.A ? plan_four_step_with(&p, ctx, st, c.A, c.G) : plan_lds_with(&p, ctx, st, &c.chain, c.G);
     if (rc != FFTSET_OK)
         return rc;
-    if (c.A && (p.fs.A != c.A || p.fs.cols.G != c.G))
+    if (c.A && (p.fs.A != c.A || (c.G && p.fs.cols.G != c.G)))
         return FFTSET_E_UNSUPPORTED;
     const long long n2 = 2LL * L;
     auto run = [&]() {
@@ -227,6 +228,11 @@ int fftset_autotune(struct fftset* fc, const struct fftset_modulation* modulatio
             }
         }
     }
+    // an LDS size that is A x 256 for a column length the column-register
+    // kernel is compiled for: that split too (it keeps the transform in
+    // VGPRs, so two workgroups share a CU where the LDS engine fits one)
+    if (!cands.empty() && complex_bins % 256 == 0 && rcol_length(complex_bins / 256))
+        cands.push_back({{}, 0, complex_bins / 256});
     if (cands.empty()) {
         std::vector<std::pair<double, unsigned>> splits;
         for (unsigned A = 2; A < complex_bins; ++A)

@@ -27,6 +27,7 @@
 #include "context.h"
 #include "fs1.h"
 #include "fsr.h"
+#include "rcol.h"
 #include "planner.h"
 #include "refpass.h"
 
@@ -62,6 +63,7 @@ struct BatchClass {
     long long pass_off = 0;        // per-pass baseline: offset of its work tiles (float2)
     float* work_base = nullptr;    // four-step: its first work row in the pool (host view)
     bool fsr_ok = false;           // a 240 x 256 four-step class the CU-resident kernel can run
+    bool rcol_ok = false;          // an A x 256 class the column-register kernel can run
 };
 
 // A contiguous run of tiles launched together with one workgroup size.
@@ -169,6 +171,7 @@ static int plan_class(fftset_batch* b, BatchClass& k, hipStream_t)
     }
     k.fs = p->fs;
     k.fsr_ok = k.fs.fsr && !b->fs_tiles;  // four-step steps as ragged tiles keep the split launches
+    k.rcol_ok = k.fs.rcol && !b->fs_tiles;
     if (k.fs.cols.E != 16 || k.fs.rows.E != 16)
         return FFTSET_E_UNSUPPORTED;
     r.A = (int)k.fs.A;
@@ -502,6 +505,12 @@ static bool fs1_class(const BatchClass& k)
            fs1_matches(k.fs.colc, k.fs.rowc, k.fs.cols.T, k.fs.rows.T);
 }
 
+// True when the class runs the column-register kernel (one launch, step 0).
+static bool rcol_class(const BatchClass& k)
+{
+    return k.rcol_ok && !generic_forced() && !persistent_mode() && !four_step_fused() && !four_step_split();
+}
+
 // True when the class runs the CU-resident kernel (one launch, step 0).
 static bool fsr_class(const BatchClass& k)
 {
@@ -550,6 +559,24 @@ static int launch_fs_class(const BatchClass& k, int q, hipStream_t st, long long
         count_launch(true);
         const int opk = r.op == OP_FWD ? FS1_FWD : r.op == OP_INV ? FS1_INV : FS1_CONV;
         return launch_fs1(r.mod, opk, f, fs1_grid(a.ntrans), st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
+    }
+    if (rcol_class(k)) {
+        // columns in registers, rows through LDS: all steps in one launch
+        if (q != 0)
+            return FFTSET_OK;
+        RcolArgs f{};
+        f.tw_row = fs.rowc.p[0].tw;
+        f.tfs = fs.tfs;
+        f.tfs_in = fo ? fs.tfs_fo : fs.tfs;
+        f.tmod = r.tmod;
+        f.ntrans = a.ntrans;
+        f.conj_kern = a.conj_kern;
+        f.in_p = r.in;
+        f.out_p = r.out;
+        f.kern_p = conv ? r.kern : nullptr;
+        count_launch(true);
+        const int opk = r.op == OP_FWD ? RC_FWD : r.op == OP_INV ? RC_INV : r.op == OP_KERN ? RC_KERN : RC_CONV;
+        return launch_rcol(r.A, r.mod, opk, f, st) == hipSuccess ? FFTSET_OK : FFTSET_E_DEVICE;
     }
     if (fsr_class(k)) {
         // the whole transform on one CU, all steps in one launch (step 0);
@@ -722,7 +749,7 @@ static int launch_stage0(const fftset_batch* b, hipStream_t st, bool branches)
     if (!fsc.empty()) {
         if (int rc = launch_fs_class(*fsc[0], 0, st))
             return rc;
-        if (!fs1_class(*fsc[0]) && !fsr_class(*fsc[0])) {  // one launch per transform: no column step to wait for
+        if (!fs1_class(*fsc[0]) && !fsr_class(*fsc[0]) && !rcol_class(*fsc[0])) {  // one launch: nothing to wait for
             if (hipEventRecord(b->mid, st) != hipSuccess)
                 return FFTSET_E_DEVICE;
             after = b->mid;

@@ -48,6 +48,10 @@ struct FourStep {
     // pass table W_240^{u k} at (k-1)*16 + u (chain [15, 16]).
     bool fsr = false;
     const float2* tw_close = nullptr;
+    // An A x 256 split with row chain [16, 16] and a compiled column length:
+    // the column-register kernel (kernels_rcol_*.hip) runs every operation,
+    // with no intermediate in HBM.
+    bool rcol = false;
 };
 
 // Autotuned choices for one (modulation, size): radix chain and packing of
@@ -129,6 +133,9 @@ int build_four_step(Ctx* ctx, unsigned L, int max_threads, FourStep* out, hipStr
 // (LDS engine) or split (four-step) instead of the defaults.
 int plan_lds_with(fftset_fft* p, Ctx* ctx, hipStream_t st, const std::vector<int>* chain, int G);
 int plan_four_step_with(fftset_fft* p, Ctx* ctx, hipStream_t st, unsigned A, int cols_C = 0);
+// True when operation `op` of four-step plan p keeps its intermediate in a
+// work region (false for plans the column-register kernel runs).
+bool four_step_work(const fftset_fft* p, int op);
 const TuneEntry* find_tune(const Ctx* ctx, int mod, unsigned L);
 int load_tune_file(Ctx* ctx, const char* path);
 

@@ -79,6 +79,53 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 __device__ __forceinline__ float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
 
+// ------------------------------------------ register and address pinning
+// Keeps a table pointer in SGPRs and its loads inside the current step (an
+// empty asm statement, no memory access): left to itself the compiler hoists
+// the twiddle loads of every chunk to the top and spills.
+template <class T>
+__device__ __forceinline__ const T* fresh(const T* p)
+{
+    // through the global address space, so loads stay global (not flat)
+    typedef const T __attribute__((address_space(1)))* gptr;
+    gptr q = (gptr)p;
+    asm volatile("" : "+s"(q));
+    return (const T*)q;
+}
+
+// Element lo of p + uni, with p + uni formed in SGPRs (uniform) and the
+// per-lane part a 32-bit offset: the access is then an SGPR base plus one
+// lane-offset VGPR shared by every access of the step. Without the asm the
+// compiler folds the lane offset into a 64-bit VGPR base first and needs a
+// VGPR pair per distinct uniform offset.
+template <class T>
+__device__ __forceinline__ T* ub(T* p, int uni, int lo)
+{
+    typedef T __attribute__((address_space(1)))* gptr;
+    gptr q = (gptr)(p + uni);
+    asm volatile("" : "+s"(q));
+    return off32((T*)q, lo);
+}
+
+// Materialises v here: values parked in VGPRs for a later phase would
+// otherwise have their last multiply sunk to the use, keeping both factors
+// live across the whole kernel.
+__device__ __forceinline__ void pin(float2& v)
+{
+    fsa_f2v t = {v.x, v.y};  // one VGPR pair, as the 64-bit loads and stores want it
+    asm volatile("" : "+v"(t));
+    v = make_float2(t.x, t.y);
+}
+
+// x, opaque to the compiler: the write-back addresses of an in-place pass
+// are recomputed from it instead of keeping the 16 read addresses live
+// across the butterfly.
+__device__ __forceinline__ int fresh_int(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // ------------------------------------------------------- compile-time loops
 template <class F, int... I>
 __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
@@ -256,5 +303,36 @@ template <> struct DFT<10> : DFTct<2, 5> {};
 template <> struct DFT<12> : DFTct<4, 3> {};
 template <> struct DFT<15> : DFTct<3, 5> {};
 template <> struct DFT<16> : DFTct<4, 4> {};
+
+// DFTct for a whole column held by one lane (rcol_impl.h): the same step,
+// with each small DFT fenced off from the machine scheduler, which would
+// otherwise interleave all of them and run out of VGPRs.
+template <int A, int B>
+struct DFTctSeq {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 t[A * B];
+        static_for<B>([&](auto N2) {
+            constexpr int n2 = decltype(N2)::value;
+            float2 c[A];
+            static_for<A>([&](auto N1) { c[N1] = v[B * decltype(N1)::value + n2]; });
+            DFT<A>::run(c);
+            static_for<A>([&](auto K1) {
+                constexpr int k1 = decltype(K1)::value;
+                t[k1 * B + n2] = rootmul<n2 * k1, A * B>(c[k1]);
+            });
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        static_for<A>([&](auto K1) {
+            constexpr int k1 = decltype(K1)::value;
+            float2 r[B];
+            static_for<B>([&](auto N2) { r[N2] = t[k1 * B + decltype(N2)::value]; });
+            DFT<B>::run(r);
+            static_for<B>([&](auto K2) { v[k1 + A * decltype(K2)::value] = r[K2]; });
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    }
+};
+template <> struct DFT<60> : DFTctSeq<4, 15> {};
 
 }  // namespace fsa

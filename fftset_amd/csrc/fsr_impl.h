@@ -46,43 +46,6 @@ constexpr int kThreads = 512;
 constexpr int kTile = 48 * kB;           // column tile: complex offset in LDS (batch rows 48..79)
 constexpr int kLds = 80 * kB * 8;        // one batch of 80 rows: 163840 bytes
 
-// Keeps a table pointer in SGPRs and its loads inside the current step (an
-// empty asm statement, no memory access): left to itself the compiler hoists
-// the twiddle loads of every chunk to the top and spills.
-template <class T>
-__device__ __forceinline__ const T* fresh(const T* p)
-{
-    // through the global address space, so loads stay global (not flat)
-    typedef const T __attribute__((address_space(1)))* gptr;
-    gptr q = (gptr)p;
-    asm volatile("" : "+s"(q));
-    return (const T*)q;
-}
-
-// Element lo of p + uni, with p + uni formed in SGPRs (uniform) and the
-// per-lane part a 32-bit offset: the access is then an SGPR base plus one
-// lane-offset VGPR shared by every access of the step. Without the asm the
-// compiler folds the lane offset into a 64-bit VGPR base first and needs a
-// VGPR pair per distinct uniform offset.
-template <class T>
-__device__ __forceinline__ T* ub(T* p, int uni, int lo)
-{
-    typedef T __attribute__((address_space(1)))* gptr;
-    gptr q = (gptr)(p + uni);
-    asm volatile("" : "+s"(q));
-    return off32((T*)q, lo);
-}
-
-// Materialises v here: values parked in VGPRs for a later phase would
-// otherwise have their last multiply sunk to the use, keeping both factors
-// live across the whole kernel.
-__device__ __forceinline__ void pin(float2& v)
-{
-    fsa_f2v t = {v.x, v.y};  // one VGPR pair, as the 64-bit loads and stores want it
-    asm volatile("" : "+v"(t));
-    v = make_float2(t.x, t.y);
-}
-
 // The thread index, opaque to the compiler: lane-dependent addresses
 // derived from it are computed where they are used instead of being hoisted
 // to the top of the kernel (dozens of address VGPRs live across every
@@ -92,15 +55,6 @@ __device__ __forceinline__ int fresh_tid()
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
     return t & (kThreads - 1);
-}
-
-// x, opaque to the compiler: the write-back addresses of an in-place pass
-// are recomputed from it instead of keeping the 16 read addresses live
-// across the butterfly.
-__device__ __forceinline__ int fresh_int(int x)
-{
-    asm volatile("" : "+v"(x));
-    return x;
 }
 
 // Column lanes: lane lt (block b = lt) of a 32-column chunk. The chunks

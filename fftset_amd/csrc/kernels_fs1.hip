@@ -64,15 +64,9 @@ struct StSlotRowTw {
 
 // The twiddle tables are the same for every chunk and transform; left to
 // itself the compiler hoists their loads out of the loops and keeps both
-// chains' twiddles live across all steps, which spills. Passing the table
-// pointers through an empty asm statement each chunk (a register-only
-// barrier: no memory access) keeps every load inside its chunk.
-template <class T>
-__device__ __forceinline__ const T* fresh(const T* p)
-{
-    asm volatile("" : "+s"(p));
-    return p;
-}
+// chains' twiddles live across all steps, which spills. The table pointers
+// pass through fresh() (dev_math.h) each chunk, which keeps every load
+// inside its chunk.
 
 template <int MOD, int OPK, int BOT>
 __global__ void __launch_bounds__(1024) k_fs1(const Fs1Args a)

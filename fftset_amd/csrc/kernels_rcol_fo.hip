@@ -1,0 +1,14 @@
+// kernels_rcol_fo.hip -- FREQOFFSETREAL instantiations of the
+// column-register four-step kernel (rcol_impl.h).
+#include "rcol_impl.h"
+
+namespace fsa {
+namespace rcol {
+
+hipError_t launch_fo(int A, int opk, const RcolArgs& a, hipStream_t st)
+{
+    return A == 60 ? launch_mod<60, MOD_FO>(opk, a, st) : hipErrorInvalidValue;
+}
+
+}  // namespace rcol
+}  // namespace fsa

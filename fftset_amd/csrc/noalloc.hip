@@ -45,7 +45,7 @@ int noalloc_work(int device, float** out)
 static size_t work_per_transform(const fftset_fft* p, int op)
 {
     const size_t len2 = 2ull * p->lfft;
-    const bool four = p->engine == ENG_FOUR_STEP;
+    const bool four = four_step_work(p, op == FFTSET_OP_GET_KERNEL ? OP_KERN : OP_CONV);
     switch (op) {
     case FFTSET_OP_GET_KERNEL: return four && p->fs.fsr ? 2ull * kFsrSpill : 0;  // the resident kernel's spill
     case FFTSET_OP_CONV_GRAD_KERNEL: return len2 * (four ? 2 : 1);
