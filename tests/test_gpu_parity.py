@@ -367,16 +367,18 @@ def test_allocation_failure_is_reported(fs):
                                       (FO, 2048, "conv"), (FO, 2048, "grad"), (FO, 2048, "kern"),
                                       (CX, 61440, "fwd"), (CX, 61440, "inv"), (FO, 61440, "fwd"),
                                       (FO, 61440, "inv"), (FO, 61440, "conv"), (CX, 61440, "grad"),
-                                      (FO, 61440, "kern")])
+                                      (FO, 61440, "kern"), (FO, 15360, "conv"), (FO, 15360, "kern")])
 def test_specialised_kernels_match_generic(tuned_fs, mod, L, op):
     # compile-time specialised chains (kernels_fixed.hip; the four-step
-    # kernels' [16,15] x [16,16] split of 61440) against the generic
+    # kernels' [16,15] x [16,16] split of 61440; the column-register kernel
+    # of the 60 x 256 split of 15360) against the generic
     # runtime-chain kernels on the same plan and inputs. The specialised
     # chains are the MI355X tune table's, so the plans come from a set that
     # loaded it.
     import torch
     f = tuned_fs.create_fft(mod, L)
-    expect = {3840: [15, 16, 16], 4096: [16, 16, 16], 2048: [8, 16, 16], 61440: [16, 15, 0, 16, 16]}[L]
+    expect = {3840: [15, 16, 16], 4096: [16, 16, 16], 2048: [8, 16, 16], 61440: [16, 15, 0, 16, 16],
+              15360: [15, 4, 0, 16, 16]}[L]
     assert f.radices() == expect
     n = 64
     rng = np.random.default_rng(L + len(op))
