@@ -65,6 +65,27 @@ struct Lay {
     static constexpr int kLds = kBufs * kBuf * 8;
 };
 
+// Row-pass twiddles W_256^{j k} (k = 1..15) of lane j from six entries of
+// the pass table tw[(k-1)*16 + j]: W^{j k} = W^{j (k & 3)} W^{4 j (k >> 2)}.
+struct RowTw {
+    float2 lo[3], hi[3];  // W^{j b}, b = 1..3; W^{4 j a}, a = 1..3
+    __device__ __forceinline__ void load(const float2* tw, int j)
+    {
+#pragma unroll
+        for (int b = 1; b < 4; ++b) lo[b - 1] = at2(tw, (b - 1) * 16, j);
+#pragma unroll
+        for (int a = 1; a < 4; ++a) hi[a - 1] = at2(tw, (4 * a - 1) * 16, j);
+    }
+    // v times W^{j k} (k a compile-time constant after unrolling, k >= 1)
+    __device__ __forceinline__ float2 apply(float2 v, int k) const
+    {
+        const int b = k & 3, a = k >> 2;
+        if (a == 0)
+            return cmul(v, lo[b - 1]);
+        return b == 0 ? cmul(v, hi[a - 1]) : cmul(v, cmul(lo[b - 1], hi[a - 1]));
+    }
+};
+
 template <int A, int MOD, int OPK, int BOT>
 __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs a)
 {
@@ -98,19 +119,32 @@ __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs
         }
     });
     DFT<A>::run(v);
-    // output twiddles, 15 table reads at a time: the compiler would issue
-    // all A of them at once otherwise (2A more VGPRs) and spill
-    static_for<NQ>([&](auto Q) {
-        constexpr int q = decltype(Q)::value;
-        const float2* tw = fresh(MOD == MOD_FO && OPK != RC_INV ? a.tfs_in : a.tfs);
-        static_for<kRows>([&](auto Rr) {
-            constexpr int k1 = kRows * q + decltype(Rr)::value;
-            if constexpr (k1 > 0 || MOD == MOD_FO)
-                v[k1] = cmul(v[k1], *ub(tw, k1 * kB, n2));
+    // output twiddles W^{n2 k1} = W^{n2 a} W^{4 n2 b} (k1 = a + 4 b) from
+    // A/4 + 3 table entries read in one go: A separate reads in dependent
+    // groups cost a round trip each. FREQOFFSETREAL's tfs_in row a carries
+    // the modulation factor W_{4n}^{n2}; row 4 b of plain tfs does not.
+    {
+        constexpr bool kFold = MOD == MOD_FO && OPK != RC_INV;
+        const float2* tin = fresh(kFold ? a.tfs_in : a.tfs);
+        const float2* tpl = fresh(a.tfs);
+        float2 lo[4], hi[A / 4];
+        static_for<4>([&](auto Aa) {
+            constexpr int aa = decltype(Aa)::value;
+            lo[aa] = (aa > 0 || kFold) ? *ub(tin, aa * kB, n2) : make_float2(1.f, 0.f);
+        });
+        static_for<A / 4>([&](auto Bb) {
+            constexpr int bb = decltype(Bb)::value;
+            hi[bb] = bb > 0 ? *ub(tpl, 4 * bb * kB, n2) : make_float2(1.f, 0.f);
+        });
+        static_for<A>([&](auto K1) {
+            constexpr int k1 = decltype(K1)::value, aa = k1 & 3, bb = k1 >> 2;
+            if constexpr (bb > 0)
+                v[k1] = cmul(v[k1], (aa > 0 || kFold) ? cmul(lo[aa], hi[bb]) : hi[bb]);
+            else if constexpr (aa > 0 || kFold)
+                v[k1] = cmul(v[k1], lo[aa]);
             pin(v[k1]);  // the product itself stays live, not its two factors
         });
-        __builtin_amdgcn_sched_barrier(0);
-    });
+    }
 
     // ------------------------------------------------------------------- rows
     static_for<NQ>([&](auto Q) {
@@ -137,9 +171,10 @@ __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs
 #pragma unroll
                 for (int s = 0; s < 16; ++s) w[s] = buf[S::pa(r, j) + 17 * s];
                 DFT<16>::run(w);
-                const float2* tw = fresh(a.tw_row);
+                RowTw tw;
+                tw.load(fresh(a.tw_row), j);
 #pragma unroll
-                for (int k = 1; k < 16; ++k) w[k] = cmul(w[k], at2(tw, (k - 1) * 16, j));
+                for (int k = 1; k < 16; ++k) w[k] = tw.apply(w[k], k);
                 const int rw = fresh_int(r), jw = fresh_int(j);
 #pragma unroll
                 for (int k = 0; k < 16; ++k) buf[S::pa(rw, jw) + 17 * k] = w[k];
@@ -207,6 +242,19 @@ __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs
                 for (int s = 0; s < 16; ++s) buf[S::pb(rw, bw) + s] = w[s];
             }
             __syncthreads();
+            // this batch's read-back twiddles W^{n2 k1} = W^{n2 a} W^{4 n2 b}
+            // (k1 = a + 4 b), issued before the DIT pass so that their
+            // latency hides behind it
+            constexpr int b0 = (kRows * q) >> 2, nb = ((kRows * q + kRows - 1) >> 2) - b0 + 1;
+            float2 rlo[3], rhi[nb];
+            {
+                const float2* tpl = fresh(a.tfs);
+                static_for<3>([&](auto Aa) { rlo[Aa] = *ub(tpl, (decltype(Aa)::value + 1) * kB, n2q); });
+                static_for<nb>([&](auto Bb) {
+                    constexpr int bb = b0 + decltype(Bb)::value;
+                    rhi[Bb] = bb ? *ub(tpl, 4 * bb * kB, n2q) : make_float2(1.f, 0.f);
+                });
+            }
             if (r < kRows) {
                 // DIT last pass: forward twiddles before a forward DFT, on
                 // conjugate data
@@ -214,9 +262,10 @@ __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs
                 float2 w[16];
 #pragma unroll
                 for (int s = 0; s < 16; ++s) w[s] = buf[S::pa(r, j) + 17 * s];
-                const float2* tw = fresh(a.tw_row);
+                RowTw tw;
+                tw.load(fresh(a.tw_row), j);
 #pragma unroll
-                for (int s = 1; s < 16; ++s) w[s] = cmul(w[s], at2(tw, (s - 1) * 16, j));
+                for (int s = 1; s < 16; ++s) w[s] = tw.apply(w[s], s);
                 DFT<16>::run(w);
                 const int rw = fresh_int(r), jw = fresh_int(j);
 #pragma unroll
@@ -224,11 +273,13 @@ __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs
             }
             __syncthreads();
             // back to the column lanes, times W_n^{n2 k1} for the closing columns
-            const float2* tw = fresh(a.tfs);
             static_for<kRows>([&](auto Rr) {
-                constexpr int r2 = decltype(Rr)::value, k1 = kRows * q + r2;
+                constexpr int r2 = decltype(Rr)::value, k1 = kRows * q + r2, aa = k1 & 3, bb = k1 >> 2;
                 const float2 y = buf[S::pos(r2, n2q)];
-                v[k1] = k1 ? cmul(y, *ub(tw, k1 * kB, n2q)) : y;
+                if constexpr (aa == 0)
+                    v[k1] = bb ? cmul(y, rhi[bb - b0]) : y;
+                else
+                    v[k1] = cmul(y, bb ? cmul(rlo[aa - 1], rhi[bb - b0]) : rlo[aa - 1]);
                 pin(v[k1]);
             });
         }
