@@ -334,5 +334,6 @@ struct DFTctSeq {
     }
 };
 template <> struct DFT<60> : DFTctSeq<4, 15> {};
+template <> struct DFT<18> : DFTctSeq<2, 9> {};
 
 }  // namespace fsa

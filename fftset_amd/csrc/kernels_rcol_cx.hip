@@ -8,7 +8,7 @@ namespace rcol {
 hipError_t launch_fo(int A, int opk, const RcolArgs& a, hipStream_t st);
 }  // namespace rcol
 
-bool rcol_length(unsigned A) { return A == 60; }
+bool rcol_length(unsigned A) { return A == 60 || A == 18; }
 
 bool rcol_matches(const ChainD& colc, const ChainD& rowc)
 {
@@ -19,11 +19,11 @@ hipError_t launch_rcol(int A, int mod, int opk, const RcolArgs& a, hipStream_t s
 {
     if (a.ntrans <= 0)
         return hipSuccess;
-    if (a.ntrans > 0x7fffffffLL || A != 60)
+    if (a.ntrans > 0x7fffffffLL || !rcol_length((unsigned)A))
         return hipErrorInvalidValue;
     if (mod == MOD_FO)
         return rcol::launch_fo(A, opk, a, st);
-    return rcol::launch_mod<60, MOD_CX>(opk, a, st);
+    return A == 60 ? rcol::launch_mod<60, MOD_CX>(opk, a, st) : rcol::launch_mod<18, MOD_CX>(opk, a, st);
 }
 
 }  // namespace fsa

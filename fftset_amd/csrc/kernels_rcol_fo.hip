@@ -7,7 +7,9 @@ namespace rcol {
 
 hipError_t launch_fo(int A, int opk, const RcolArgs& a, hipStream_t st)
 {
-    return A == 60 ? launch_mod<60, MOD_FO>(opk, a, st) : hipErrorInvalidValue;
+    return A == 60 ? launch_mod<60, MOD_FO>(opk, a, st)
+         : A == 18 ? launch_mod<18, MOD_FO>(opk, a, st)
+                   : hipErrorInvalidValue;
 }
 
 }  // namespace rcol

@@ -8,8 +8,9 @@
 // (element n1 of column n2 is element n2 + 256 n1), 2A VGPRs of data.
 //   Columns: a length-A DFT per lane in registers (no LDS, no barrier),
 //     inputs read as 256-lane rows (2 KB runs), outputs times W_n^{n2 k1}.
-//   Rows: batches of 15 rows through a 33-35 KB LDS buffer (three
-//     workgroups share a CU, at <= 168 VGPRs): the radix-16 DIF pass
+//   Rows: batches of 15 rows (A = 60; 9 for A = 18) through a 33-35 KB
+//     (20-21 KB) LDS buffer, so that three (five) workgroups share a CU at
+//     <= 168 (102) VGPRs: the radix-16 DIF pass
 //     over elements j + 16 s with twiddles W_256^{j k}, then the radix-16
 //     pass over 16 b + s. Forward / inverse store value k of block b, bin
 //     k2 = b + 16 k of row k1, at X[k1 + A k2] (lanes across rows);
@@ -34,12 +35,24 @@ namespace rcol {
 
 constexpr int kB = 256;
 constexpr int kThreads = 256;
-constexpr int kRows = 15;  // rows per batch
+// Rows per LDS batch: 15 for A = 60 (four batches); 9 for A = 18 (two).
+template <int A>
+constexpr int rows_of()
+{
+    return A % 15 == 0 ? 15 : A % 9 == 0 ? 9 : 1;
+}
 // LDS buffers for the row batches: with one, a batch waits for the previous
 // batch's last LDS reads before it writes (one more barrier per batch), and
 // three workgroups share a CU (LDS and <= 168 VGPRs); with two, two do.
 constexpr int kBufs = 1;
-constexpr int kWavesPerSimd = kBufs == 1 ? 3 : 2;
+// Workgroups per CU (= waves per SIMD) the register budget is set for: a
+// 60-value column leaves room for three; an 18-value one for five, or four
+// for the convolution (its spectrum and the held rows: 128 VGPRs).
+template <int A, bool CONV>
+constexpr int waves_of()
+{
+    return A <= 18 ? (CONV ? 4 : 5) : kBufs == 1 ? 3 : 2;
+}
 
 // LDS position of element p of batch row r: one padding slot per 16
 // elements, rows RS apart. Every access of a pass is then one lane base plus
@@ -52,7 +65,7 @@ constexpr int kWavesPerSimd = kBufs == 1 ? 3 : 2;
 //   lanes across rows, (r, j) = (tid & 15, tid >> 4), over j + 16 s or
 //     16 j + s: RS = 290 = 2 (mod 32) spreads the 16 rows over the even
 //     (or odd) bank pairs.
-template <bool ACROSS>
+template <bool ACROSS, int kRows>
 struct Lay {
     static constexpr int RS = ACROSS ? 290 : 272;
     static constexpr int kBuf = kRows * RS;  // complex values per LDS buffer
@@ -87,8 +100,9 @@ struct RowTw {
 };
 
 template <int A, int MOD, int OPK, int BOT>
-__global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs a)
+__global__ void __launch_bounds__(kThreads, (waves_of<A, OPK == RC_CONV>())) k_rcol(const RcolArgs a)
 {
+    constexpr int kRows = rows_of<A>();
     constexpr int N = A * kB, NQ = A / kRows;
     static_assert(A % kRows == 0, "whole row batches");
     constexpr bool kConv = OPK == RC_CONV, kKern = OPK == RC_KERN;
@@ -127,12 +141,12 @@ __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs
         constexpr bool kFold = MOD == MOD_FO && OPK != RC_INV;
         const float2* tin = fresh(kFold ? a.tfs_in : a.tfs);
         const float2* tpl = fresh(a.tfs);
-        float2 lo[4], hi[A / 4];
+        float2 lo[4], hi[(A + 3) / 4];
         static_for<4>([&](auto Aa) {
             constexpr int aa = decltype(Aa)::value;
             lo[aa] = (aa > 0 || kFold) ? *ub(tin, aa * kB, n2) : make_float2(1.f, 0.f);
         });
-        static_for<A / 4>([&](auto Bb) {
+        static_for<(A + 3) / 4>([&](auto Bb) {
             constexpr int bb = decltype(Bb)::value;
             hi[bb] = bb > 0 ? *ub(tpl, 4 * bb * kB, n2) : make_float2(1.f, 0.f);
         });
@@ -149,7 +163,7 @@ __global__ void __launch_bounds__(kThreads, kWavesPerSimd) k_rcol(const RcolArgs
     // ------------------------------------------------------------------- rows
     static_for<NQ>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
-        using S = Lay<kAcross>;
+        using S = Lay<kAcross, kRows>;
         float2* buf = lds + (q % kBufs) * S::kBuf;
         // this batch's rows into LDS: one buffer is free once every lane has
         // read the previous batch; of two, the other buffer's last readers
@@ -324,7 +338,8 @@ hipError_t launch_mod(int opk, const RcolArgs& a, hipStream_t st)
         break;
     default: return hipErrorInvalidValue;
     }
-    const int lds = opk == RC_FWD || opk == RC_INV ? Lay<true>::kLds : Lay<false>::kLds;
+    constexpr int R = rows_of<A>();
+    const int lds = opk == RC_FWD || opk == RC_INV ? Lay<true, R>::kLds : Lay<false, R>::kLds;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipGetLastError();  // report this launch, not an earlier failed call

@@ -1,6 +1,6 @@
-"""The column-register four-step kernel (kernels_rcol_*.hip): 15360-bin
-transforms as a 60 x 256 split whose columns each live in one lane's VGPRs
-and whose rows pass through LDS in batches of 15. Checked against the
+"""The column-register four-step kernel (kernels_rcol_*.hip): 15360- and
+4608-bin transforms as 60 x 256 and 18 x 256 splits whose columns each live
+in one lane's VGPRs and whose rows pass through LDS in batches of 15 and 9. Checked against the
 two-launch four-step path of the same split (FFTSET_SELECT_FOUR_STEP_SPLIT:
 same decomposition, twiddle tables and spectrum layout, so equal to
 rounding), against the LDS engine's plan of the same size (a different
@@ -21,7 +21,7 @@ from tests.gpu_util import torch_cuda
 pytestmark = pytest.mark.gpu
 
 FO, CX = fftset_amd.FREQ_OFFSET_REAL, fftset_amd.COMPLEX
-L = 15360
+SIZES = [15360, 4608]  # 60 x 256, 18 x 256
 OPS = {"fwd": fftset_amd.OP_FORWARD, "inv": fftset_amd.OP_INVERSE, "kern": fftset_amd.OP_GET_KERNEL,
        "conv": fftset_amd.OP_CONV, "grad": fftset_amd.OP_CONV_GRAD_INPUT}
 
@@ -30,14 +30,18 @@ def fp64_tol(n):
     return 1e-6 * np.log2(n)
 
 
+def _decay(L):
+    return np.exp(-np.arange(2 * L) / (L / 15.0))
+
+
 @pytest.fixture(scope="module")
 def fs(tmp_path_factory):
-    """A set whose 15360-bin plans are the 60 x 256 split (a tune table
-    entry with A = 60 and no chain)."""
+    """A set whose 15360- and 4608-bin plans are the A x 256 splits (tune
+    table entries with A and no chain)."""
     if torch_cuda() is None:
         pytest.skip("no GPU")
     path = tmp_path_factory.mktemp("rcol") / "tune.txt"
-    path.write_text(f"1 {L} 0 60 0 0\n0 {L} 0 60 0 0\n")
+    path.write_text("".join(f"{m} {n} 0 {n // 256} 0 0\n" for n in SIZES for m in (0, 1)))
     s = fftset_amd.FFTSet(tune_file=str(path))
     yield s
     s.destroy()
@@ -52,11 +56,10 @@ def fs_lds():
     s.destroy()
 
 
-def _data(n, seed):
+def _data(L, n, seed):
     rng = np.random.default_rng(seed)
     x = torch.from_numpy(rng.standard_normal((n, 2 * L)).astype(np.float32)).cuda()
-    h = torch.from_numpy((rng.standard_normal((n, 2 * L)) *
-                          np.exp(-np.arange(2 * L) / 1024.0)).astype(np.float32)).cuda()
+    h = torch.from_numpy((rng.standard_normal((n, 2 * L)) * _decay(L)).astype(np.float32)).cuda()
     return x, h
 
 
@@ -85,20 +88,22 @@ def _fp64(mod, op, xs, hs):
     return None
 
 
-def test_rcol_plan(fs):
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_plan(fs, L):
     for mod in (CX, FO):
         r = fs.create_fft(mod, L).radices()
         assert r[-3:] == [0, 16, 16], r
-        assert int(np.prod(r[:-3])) == 60, r
+        assert int(np.prod(r[:-3])) == L // 256, r
 
 
 @pytest.mark.parametrize("mod,op", [(CX, "fwd"), (CX, "inv"), (FO, "fwd"), (FO, "inv"), (CX, "kern"),
                                     (FO, "kern"), (CX, "conv"), (FO, "conv"), (CX, "grad"), (FO, "grad")])
-def test_rcol_equals_split_launches(fs, mod, op):
-    # 600 transforms: more than two workgroups per CU over the grid
-    n = 600
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_equals_split_launches(fs, mod, op, L):
+    # 1500 transforms: more than five workgroups per CU over the grid
+    n = 1500
     f = fs.create_fft(mod, L)
-    x, h = _data(n, 7 + len(op) + len(mod))
+    x, h = _data(L, n, 7 + len(op) + len(mod))
     k = torch.empty_like(h)
     f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
     torch.cuda.synchronize()
@@ -118,12 +123,13 @@ def test_rcol_equals_split_launches(fs, mod, op):
 
 
 @pytest.mark.parametrize("mod", [CX, FO])
-def test_rcol_equals_lds_plan(fs, fs_lds, mod):
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_equals_lds_plan(fs, fs_lds, mod, L):
     # layout-free results agree with the LDS engine's plan of the same size
     n = 8
     f, g = fs.create_fft(mod, L), fs_lds.create_fft(mod, L)
     assert g.radices()[-3:] != [0, 16, 16]
-    x, h = _data(n, 19 + len(mod))
+    x, h = _data(L, n, 19 + len(mod))
     kf, kg = torch.empty_like(h), torch.empty_like(h)
     f.execute(fftset_amd.OP_GET_KERNEL, n, kf, h)
     g.execute(fftset_amd.OP_GET_KERNEL, n, kg, h)
@@ -134,10 +140,11 @@ def test_rcol_equals_lds_plan(fs, fs_lds, mod):
 
 
 @pytest.mark.parametrize("mod", [FO, CX])
-def test_rcol_spectra_interchange_with_split_launches(fs, mod):
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_spectra_interchange_with_split_launches(fs, mod, L):
     n = 4
     f = fs.create_fft(mod, L)
-    x, h = _data(n, 31 + len(mod))
+    x, h = _data(L, n, 31 + len(mod))
     k_res = torch.empty_like(h)
     f.execute(fftset_amd.OP_GET_KERNEL, n, k_res, h)
     k_split = torch.empty_like(h)
@@ -151,7 +158,7 @@ def test_rcol_spectra_interchange_with_split_launches(fs, mod):
         assert np.max(fp64.rel_rms(y.cpu().numpy(), ref)) <= fp64_tol(L)
 
 
-def _at(n, off, seed, scale=None):
+def _at(L, n, off, seed, scale=None):
     # n transforms back to back starting `off` floats into a fresh buffer
     rng = np.random.default_rng(seed)
     v = rng.standard_normal(n * 2 * L + 64)
@@ -169,22 +176,22 @@ def _run_into(f, op, y, x, k=None):
 
 @pytest.mark.parametrize("mod", [CX, FO])
 @pytest.mark.parametrize("off_in,off_out", [(2, 2), (16, 0), (0, 26)])
-def test_rcol_misaligned_buffers(fs, mod, off_in, off_out):
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_misaligned_buffers(fs, mod, off_in, off_out, L):
     n = 3
     f = fs.create_fft(mod, L)
-    decay = np.exp(-np.arange(2 * L) / 1024.0)
-    x = _at(n, off_in, 70 + off_in)
-    h = _at(n, off_in, 71 + off_in, decay)
-    k = _at(n, off_in, 72)
+    x = _at(L, n, off_in, 70 + off_in)
+    h = _at(L, n, off_in, 71 + off_in, _decay(L))
+    k = _at(L, n, off_in, 72)
     f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
     torch.cuda.synchronize()
-    ks = _split(lambda: _run_into(f, "kern", _at(n, off_out, 73), h))
+    ks = _split(lambda: _run_into(f, "kern", _at(L, n, off_out, 73), h))
     assert np.max(fp64.rel_rms(k.cpu().numpy(), ks.cpu().numpy())) <= 1e-6
     xs, hs = x.cpu().numpy(), h.cpu().numpy()
     for op in ("fwd", "inv", "conv", "grad"):
         kk = k if op in ("conv", "grad") else None
-        y = _run_into(f, op, _at(n, off_out, 74), x, kk)
-        ys = _split(lambda: _run_into(f, op, _at(n, off_out, 75), x, kk))
+        y = _run_into(f, op, _at(L, n, off_out, 74), x, kk)
+        ys = _split(lambda: _run_into(f, op, _at(L, n, off_out, 75), x, kk))
         r = y.cpu().numpy()
         assert np.all(np.isfinite(r)), op
         assert np.max(fp64.rel_rms(r, ys.cpu().numpy())) <= 1e-6, op
@@ -194,12 +201,13 @@ def test_rcol_misaligned_buffers(fs, mod, off_in, off_out):
 
 
 @pytest.mark.parametrize("mod,op", [(CX, "fwd"), (FO, "inv"), (FO, "conv"), (CX, "kern")])
-def test_rcol_in_place(fs, mod, op):
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_in_place(fs, mod, op, L):
     # out == in: every lane has read its whole column before the first
     # barrier, and outputs are written only after it
     n = 3
     f = fs.create_fft(mod, L)
-    x, h = _data(n, 41)
+    x, h = _data(L, n, 41)
     k = torch.empty_like(h)
     f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
     ref = _run(f, op, x, k if op == "conv" else None).cpu().numpy()
@@ -209,11 +217,12 @@ def test_rcol_in_place(fs, mod, op):
     assert np.array_equal(y.cpu().numpy(), ref)
 
 
-def test_rcol_shared_spectrum(fs):
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_shared_spectrum(fs, L):
     # one spectrum for every transform (kernel stride 0): cached loads
     n = 5
     f = fs.create_fft(FO, L)
-    x, h = _data(n, 47)
+    x, h = _data(L, n, 47)
     k = torch.empty_like(h[:1])
     f.execute(fftset_amd.OP_GET_KERNEL, 1, k, h[:1])
     y = torch.full_like(x, float("nan"))
@@ -223,10 +232,11 @@ def test_rcol_shared_spectrum(fs):
     assert np.max(fp64.rel_rms(y.cpu().numpy(), ref)) <= fp64_tol(L)
 
 
-def test_rcol_save_spectrum_and_kernel_gradient(fs):
+@pytest.mark.parametrize("L", SIZES)
+def test_rcol_save_spectrum_and_kernel_gradient(fs, L):
     n = 2
     f = fs.create_fft(FO, L)
-    x, h = _data(n, 53)
+    x, h = _data(L, n, 53)
     w = torch.from_numpy(np.random.default_rng(54).standard_normal((n, 2 * L)).astype(np.float32)).cuda()
     k = torch.empty_like(h)
     f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
@@ -250,9 +260,9 @@ def test_rcol_save_spectrum_and_kernel_gradient(fs):
 def test_rcol_noalloc_needs_no_work(fs):
     # no intermediate in HBM: the no-allocation entry point takes counts
     # beyond what its 64 MiB work array could hold for a four-step plan
-    n = 600  # 600 x 2L floats = 70 MiB of work rows on the two-launch path
+    L, n = 15360, 600  # 600 x 2L floats = 70 MiB of work rows on the two-launch path
     f = fs.create_fft(CX, L)
-    x, h = _data(n, 61)
+    x, h = _data(L, n, 61)
     k = torch.empty_like(h)
     f.execute(fftset_amd.OP_GET_KERNEL, n, k, h)
     y1 = _run(f, "conv", x, k)
@@ -263,14 +273,15 @@ def test_rcol_noalloc_needs_no_work(fs):
 
 
 def test_rcol_ragged_batch_equals_drop_in(fs):
-    # a ragged batch whose 15360-bin classes run the column-register kernel
+    L = 15360
+    # a ragged batch whose 15360- and 4608-bin classes run the column-register kernel
     # with per-transform pointers, beside LDS classes: their outputs equal
     # the drop-in calls bit for bit (the LDS classes run the ragged kernel,
     # equal to rounding)
     rng = np.random.default_rng(5)
     spec = [(CX, L, fftset_amd.OP_FORWARD, 5), (FO, L, fftset_amd.OP_CONV, 6), (FO, L, fftset_amd.OP_INVERSE, 2),
             (CX, L, fftset_amd.OP_CONV_GRAD_INPUT, 2), (FO, 4608, fftset_amd.OP_CONV, 4),
-            (CX, 48, fftset_amd.OP_FORWARD, 9)]
+            (CX, 4608, fftset_amd.OP_FORWARD, 3), (CX, 48, fftset_amd.OP_FORWARD, 9), (FO, 768, fftset_amd.OP_CONV, 5)]
     b = fs.batch()
     jobs = []
     for mod, n, op, count in spec:
@@ -293,7 +304,7 @@ def test_rcol_ragged_batch_equals_drop_in(fs):
         ref = torch.full_like(x, float("nan"))
         f.execute(op, 1, ref, x, kern=k)
         torch.cuda.synchronize()
-        if f.L == L:
+        if f.L in SIZES:
             assert np.array_equal(y.cpu().numpy(), ref.cpu().numpy()), (f.L, op)
         else:
             assert fp64.rel_rms(y.cpu().numpy(), ref.cpu().numpy()) <= 2e-6, (f.L, op)
