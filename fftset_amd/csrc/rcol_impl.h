@@ -161,6 +161,12 @@ __global__ void __launch_bounds__(kThreads, (waves_of<A, OPK == RC_CONV>())) k_r
     }
 
     // ------------------------------------------------------------------- rows
+    // COMPLEX forward / inverse keep each batch's results in VGPRs (the
+    // batch's inputs have left them) and write the transform at the end in
+    // contiguous chunks staged through the LDS buffer: stored straight from
+    // the second pass, a batch's bins are runs of only kRows values.
+    constexpr bool kStage = kAcross && MOD == MOD_CX;
+    float2 res[kStage ? NQ : 1][16];
     static_for<NQ>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
         using S = Lay<kAcross, kRows>;
@@ -204,9 +210,16 @@ __global__ void __launch_bounds__(kThreads, (waves_of<A, OPK == RC_CONV>())) k_r
 #pragma unroll
                 for (int s = 0; s < 16; ++s) w[s] = buf[S::pb(r, b) + s];
                 DFT<16>::run(w);
+                if constexpr (kStage) {
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        res[kStage ? q : 0][k] = OPK == RC_INV ? cconj(w[k]) : w[k];
+                        pin(res[kStage ? q : 0][k]);
+                    }
+                }
                 const int f0 = kRows * q + r + A * b;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
+                for (int k = 0; k < 16 && !kStage; ++k) {
                     const int f = f0 + 16 * A * k;
                     if constexpr (MOD == MOD_FO && OPK == RC_FWD)
                         StFoBins{(float2*)out, N}(f, w[k]);
@@ -298,6 +311,35 @@ __global__ void __launch_bounds__(kThreads, (waves_of<A, OPK == RC_CONV>())) k_r
             });
         }
     });
+
+    if constexpr (kStage) {
+        // bins k2 in [64 c, 64 c + 64) of every row are the A * 64 values
+        // X[A * 64 c ...] in a row: lane (r, b) holds bins b + 16 k, so
+        // chunk c takes its k = 4 c .. 4 c + 3 of every batch
+        constexpr int kChunk = A * 64, kPer = (kChunk + kThreads - 1) / kThreads;
+        static_assert(kChunk <= Lay<true, kRows>::kBuf, "a chunk fits the LDS buffer");
+        const int tq = fresh_int(tid), r = tq & 15, b = tq >> 4;
+        static_for<4>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            __syncthreads();  // every lane has read the buffer's previous contents
+            if (r < kRows) {
+                static_for<NQ>([&](auto Q) {
+                    constexpr int q = decltype(Q)::value;
+                    static_for<4>([&](auto Kk) {
+                        constexpr int kk = decltype(Kk)::value;
+                        lds[kRows * q + r + A * (b + 16 * kk)] = res[q][4 * c + kk];
+                    });
+                });
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int idx = tid + kThreads * i;
+                if (kChunk % kThreads == 0 || idx < kChunk)
+                    stg(ub((float2*)out, kChunk * c, idx), lds[idx]);
+            }
+        });
+    }
 
     // -------------------------------------------------------- closing columns
     if constexpr (kConv) {
