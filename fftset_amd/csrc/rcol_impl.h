@@ -8,8 +8,8 @@
 // (element n1 of column n2 is element n2 + 256 n1), 2A VGPRs of data.
 //   Columns: a length-A DFT per lane in registers (no LDS, no barrier),
 //     inputs read as 256-lane rows (2 KB runs), outputs times W_n^{n2 k1}.
-//   Rows: batches of 15 rows (A = 60; 9 for A = 18) through a 33-35 KB
-//     (20-21 KB) LDS buffer, so that three (five) workgroups share a CU at
+//   Rows: batches of 15 rows (A = 60; 9 for A = 18) through a 33 KB
+//     (20 KB) LDS buffer, so that three (five) workgroups share a CU at
 //     <= 168 (102) VGPRs: the radix-16 DIF pass
 //     over elements j + 16 s with twiddles W_256^{j k}, then the radix-16
 //     pass over 16 b + s. Forward / inverse store value k of block b, bin
@@ -56,18 +56,18 @@ constexpr int waves_of()
 
 // LDS position of element p of batch row r: one padding slot per 16
 // elements, rows RS apart. Every access of a pass is then one lane base plus
-// a compile-time offset (17 per step of the pass), and each pattern below
-// puts at most two lanes on a bank pair (the minimum for 64-bit accesses of
-// 64 lanes; 32 bank pairs):
-//   column lanes writing / reading row r: p = n2, consecutive lanes;
+// a compile-time offset (17 per step of the pass). The LDS serves a 64-bit
+// access 16 lanes at a time over 16 bank pairs, so each group of 16 lanes
+// must touch 16 different pairs:
+//   column lanes writing / reading row r: p = n2, 16 consecutive lanes;
 //   lanes along rows, (r, j) = (tid >> 4, tid & 15), over j + 16 s or
-//     16 j + s: RS = 272 = 16 (mod 32) separates a wave's row pairs;
+//     16 j + s: a group is one row, whatever RS (272);
 //   lanes across rows, (r, j) = (tid & 15, tid >> 4), over j + 16 s or
-//     16 j + s: RS = 290 = 2 (mod 32) spreads the 16 rows over the even
-//     (or odd) bank pairs.
+//     16 j + s: a group is 16 rows, which RS = 273 = 1 (mod 16) spreads over
+//     all pairs (and 273 = 17 (mod 32) keeps two such groups apart).
 template <bool ACROSS, int kRows>
 struct Lay {
-    static constexpr int RS = ACROSS ? 290 : 272;
+    static constexpr int RS = ACROSS ? 273 : 272;
     static constexpr int kBuf = kRows * RS;  // complex values per LDS buffer
     // element p of row r
     static __device__ __forceinline__ int pos(int r, int p) { return r * RS + p + (p >> 4); }
