@@ -8,7 +8,7 @@ namespace rcol {
 hipError_t launch_fo(int A, int opk, const RcolArgs& a, hipStream_t st);
 }  // namespace rcol
 
-bool rcol_length(unsigned A) { return A == 60 || A == 18; }
+bool rcol_length(unsigned A) { return A == 60 || A == 18 || A == 16 || A == 15 || A == 8 || A == 3; }
 
 bool rcol_matches(const ChainD& colc, const ChainD& rowc)
 {
@@ -23,7 +23,12 @@ hipError_t launch_rcol(int A, int mod, int opk, const RcolArgs& a, hipStream_t s
         return hipErrorInvalidValue;
     if (mod == MOD_FO)
         return rcol::launch_fo(A, opk, a, st);
-    return A == 60 ? rcol::launch_mod<60, MOD_CX>(opk, a, st) : rcol::launch_mod<18, MOD_CX>(opk, a, st);
+    return A == 60   ? rcol::launch_mod<60, MOD_CX>(opk, a, st)
+         : A == 18 ? rcol::launch_mod<18, MOD_CX>(opk, a, st)
+         : A == 16 ? rcol::launch_mod<16, MOD_CX>(opk, a, st)
+         : A == 15 ? rcol::launch_mod<15, MOD_CX>(opk, a, st)
+         : A == 8  ? rcol::launch_mod<8, MOD_CX>(opk, a, st)
+                   : rcol::launch_mod<3, MOD_CX>(opk, a, st);
 }
 
 }  // namespace fsa

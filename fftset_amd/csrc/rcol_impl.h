@@ -35,11 +35,12 @@ namespace rcol {
 
 constexpr int kB = 256;
 constexpr int kThreads = 256;
-// Rows per LDS batch: 15 for A = 60 (four batches); 9 for A = 18 (two).
+// Rows per LDS batch: 15 for A = 60 (four batches); 9 for A = 18 (two);
+// all of them for A <= 16.
 template <int A>
 constexpr int rows_of()
 {
-    return A % 15 == 0 ? 15 : A % 9 == 0 ? 9 : 1;
+    return A % 15 == 0 ? 15 : A % 16 == 0 ? 16 : A % 9 == 0 ? 9 : A <= 16 ? A : 1;
 }
 // LDS buffers for the row batches: with one, a batch waits for the previous
 // batch's last LDS reads before it writes (one more barrier per batch), and
@@ -144,7 +145,8 @@ __global__ void __launch_bounds__(kThreads, (waves_of<A, OPK == RC_CONV>())) k_r
         float2 lo[4], hi[(A + 3) / 4];
         static_for<4>([&](auto Aa) {
             constexpr int aa = decltype(Aa)::value;
-            lo[aa] = (aa > 0 || kFold) ? *ub(tin, aa * kB, n2) : make_float2(1.f, 0.f);
+            // (rows a < A of the A x 256 tables only)
+            lo[aa] = (aa > 0 || kFold) && aa < A ? *ub(tin, aa * kB, n2) : make_float2(1.f, 0.f);
         });
         static_for<(A + 3) / 4>([&](auto Bb) {
             constexpr int bb = decltype(Bb)::value;
@@ -276,7 +278,10 @@ __global__ void __launch_bounds__(kThreads, (waves_of<A, OPK == RC_CONV>())) k_r
             float2 rlo[3], rhi[nb];
             {
                 const float2* tpl = fresh(a.tfs);
-                static_for<3>([&](auto Aa) { rlo[Aa] = *ub(tpl, (decltype(Aa)::value + 1) * kB, n2q); });
+                static_for<3>([&](auto Aa) {
+                    constexpr int aa = decltype(Aa)::value + 1;
+                    rlo[Aa] = aa < A ? *ub(tpl, aa * kB, n2q) : make_float2(1.f, 0.f);
+                });
                 static_for<nb>([&](auto Bb) {
                     constexpr int bb = b0 + decltype(Bb)::value;
                     rhi[Bb] = bb ? *ub(tpl, 4 * bb * kB, n2q) : make_float2(1.f, 0.f);

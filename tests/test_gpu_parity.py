@@ -44,6 +44,20 @@ def tuned_fs():
     s.destroy()
 
 
+@pytest.fixture(scope="module")
+def lds_tuned_fs(tmp_path_factory):
+    """A set planning 3840, 4096 and 2048 on the LDS chains the compile-time
+    three-pass kernels are specialised for (the MI355X table plans these
+    sizes on the column-register kernel instead)."""
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    path = tmp_path_factory.mktemp("lds_tune") / "tune.txt"
+    path.write_text("1 3840 1 0 0 3 15 16 16\n0 4096 1 0 0 3 16 16 16\n0 2048 2 0 0 3 8 16 16\n")
+    s = fftset_amd.FFTSet(tune_file=str(path))
+    yield s
+    s.destroy()
+
+
 FO, CX = fftset_amd.FREQ_OFFSET_REAL, fftset_amd.COMPLEX
 
 
@@ -363,22 +377,27 @@ def test_allocation_failure_is_reported(fs):
     assert fp64.rel_rms(gpu_forward(f, v), fp64.cx_forward64(v)) <= fp64_tol(L)
 
 
-@pytest.mark.parametrize("mod,L,op", [(CX, 3840, "fwd"), (CX, 3840, "inv"), (FO, 4096, "conv"), (FO, 4096, "kern"),
-                                      (FO, 2048, "conv"), (FO, 2048, "grad"), (FO, 2048, "kern"),
-                                      (CX, 61440, "fwd"), (CX, 61440, "inv"), (FO, 61440, "fwd"),
-                                      (FO, 61440, "inv"), (FO, 61440, "conv"), (CX, 61440, "grad"),
-                                      (FO, 61440, "kern"), (FO, 15360, "conv"), (FO, 15360, "kern")])
-def test_specialised_kernels_match_generic(tuned_fs, mod, L, op):
-    # compile-time specialised chains (kernels_fixed.hip; the four-step
-    # kernels' [16,15] x [16,16] split of 61440; the column-register kernel
-    # of the 60 x 256 split of 15360) against the generic
+@pytest.mark.parametrize("lds,mod,L,op", [
+    (True, CX, 3840, "fwd"), (True, CX, 3840, "inv"), (True, FO, 4096, "conv"), (True, FO, 4096, "kern"),
+    (True, FO, 2048, "conv"), (True, FO, 2048, "grad"), (True, FO, 2048, "kern"),
+    (False, CX, 61440, "fwd"), (False, CX, 61440, "inv"), (False, FO, 61440, "fwd"), (False, FO, 61440, "inv"),
+    (False, FO, 61440, "conv"), (False, CX, 61440, "grad"), (False, FO, 61440, "kern"),
+    (False, FO, 15360, "conv"), (False, FO, 15360, "kern"), (False, CX, 3840, "fwd"), (False, FO, 4096, "conv"),
+    (False, FO, 2048, "grad"), (False, CX, 4608, "fwd")])
+def test_specialised_kernels_match_generic(tuned_fs, lds_tuned_fs, lds, mod, L, op):
+    # compile-time specialised chains (kernels_fixed.hip, on a set planning
+    # their chains; the four-step kernels' [16,15] x [16,16] split of 61440;
+    # the column-register kernel of the A x 256 splits) against the generic
     # runtime-chain kernels on the same plan and inputs. The specialised
     # chains are the MI355X tune table's, so the plans come from a set that
     # loaded it.
     import torch
-    f = tuned_fs.create_fft(mod, L)
-    expect = {3840: [15, 16, 16], 4096: [16, 16, 16], 2048: [8, 16, 16], 61440: [16, 15, 0, 16, 16],
-              15360: [15, 4, 0, 16, 16]}[L]
+    f = (lds_tuned_fs if lds else tuned_fs).create_fft(mod, L)
+    if lds:
+        expect = {3840: [15, 16, 16], 4096: [16, 16, 16], 2048: [8, 16, 16]}[L]
+    else:
+        expect = {61440: [16, 15, 0, 16, 16], 15360: [15, 4, 0, 16, 16], 3840: [15, 0, 16, 16],
+                  4096: [16, 0, 16, 16], 2048: [8, 0, 16, 16], 4608: [9, 2, 0, 16, 16]}[L]
     assert f.radices() == expect
     n = 64
     rng = np.random.default_rng(L + len(op))

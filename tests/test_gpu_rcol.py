@@ -21,7 +21,7 @@ from tests.gpu_util import torch_cuda
 pytestmark = pytest.mark.gpu
 
 FO, CX = fftset_amd.FREQ_OFFSET_REAL, fftset_amd.COMPLEX
-SIZES = [15360, 4608]  # 60 x 256, 18 x 256
+SIZES = [15360, 4608, 4096, 3840, 2048, 768]  # 60, 18, 16, 15, 8 and 3 x 256
 OPS = {"fwd": fftset_amd.OP_FORWARD, "inv": fftset_amd.OP_INVERSE, "kern": fftset_amd.OP_GET_KERNEL,
        "conv": fftset_amd.OP_CONV, "grad": fftset_amd.OP_CONV_GRAD_INPUT}
 
