@@ -8,16 +8,18 @@
 // (element n1 of column n2 is element n2 + 256 n1), 2A VGPRs of data.
 //   Columns: a length-A DFT per lane in registers (no LDS, no barrier),
 //     inputs read as 256-lane rows (2 KB runs), outputs times W_n^{n2 k1}.
-//   Rows: batches of 15 rows (A = 60; 9 for A = 18) through a 33 KB
-//     (20 KB) LDS buffer, so that three (five) workgroups share a CU at
-//     <= 168 (102) VGPRs: the radix-16 DIF pass
-//     over elements j + 16 s with twiddles W_256^{j k}, then the radix-16
-//     pass over 16 b + s. Forward / inverse store value k of block b, bin
-//     k2 = b + 16 k of row k1, at X[k1 + A k2] (lanes across rows);
-//     get_kernel stores the slot-major spectrum (value k of block b of row
-//     k1 at k1*256 + 16 k + b); convolution multiplies by the spectrum,
-//     runs the DIT back (conjugate data, forward twiddles) and returns the
-//     rows to their column lanes.
+//   Rows: batches of 15 rows (A = 60; 9 for A = 18; all A rows for
+//     A <= 16) through one LDS buffer (33 KB at A = 60), so that three
+//     (A = 60) to five workgroups share a CU: the radix-16 DIF pass over
+//     elements j + 16 s with twiddles W_256^{j k}, then the radix-16 pass
+//     over 16 b + s, which yields bin k2 = b + 16 k of row k1, X[k1 + A k2].
+//     COMPLEX forward / inverse keep those bins in VGPRs and write the
+//     transform at the end in contiguous chunks staged through the buffer;
+//     FREQOFFSETREAL forward / inverse store each bin through the bin
+//     interleave; get_kernel stores the slot-major spectrum (value k of
+//     block b of row k1 at k1*256 + 16 k + b); convolution multiplies by
+//     the spectrum, runs the DIT back (conjugate data, forward twiddles) and
+//     returns the rows to their column lanes.
 //   Closing columns (convolution): a second length-A DFT per lane in
 //     registers; the output goes out as 256-lane rows in natural order.
 // This is the decomposition, the twiddle tables and the spectrum layout of
