@@ -157,13 +157,19 @@ static int plan_class(fftset_batch* b, BatchClass& k, hipStream_t)
     if (!k.four_step) {
         if (p->shape.E != 16)
             return FFTSET_E_UNSUPPORTED;
+        // the plan's own packing (its autotuned G where the tune table has
+        // one, as the uniform launch uses) in the smallest workgroup of 128
+        // or more lanes that holds it: 1440 as one transform per 128 lanes
+        // ran 1.5x faster than two per 256. FFTSET_BATCH_THREADS_512: as
+        // many transforms as 512 lanes hold.
         const int T = p->shape.T;
-        k.threads = round_threads(T, b->threads);
+        const bool wide = b->threads >= 512;
+        const long long lds_one = (long long)lds_len(r.L) * 8;
+        int G = wide ? std::max(1, round_threads(T, 512) / T) : std::max(1, p->shape.G);
+        while (G > 1 && (G * lds_one > 64 * 1024 || G * T > 1024 || G > (int)k.jobs.size())) --G;
+        k.threads = round_threads(G * T, wide ? 512 : 128);
         if (k.threads < 0)
             return FFTSET_E_UNSUPPORTED;
-        const long long lds_one = (long long)lds_len(r.L) * 8;
-        int G = std::max(1, k.threads / T);
-        while (G > 1 && (G * lds_one > 64 * 1024 || G > (int)k.jobs.size())) --G;
         r.T = T;
         r.G = G;
         r.c = p->chain;
@@ -357,14 +363,17 @@ static int commit(fftset_batch* b, hipStream_t st)
     // fused tables: LDS tiles (all in phase 0) and four-step tiles, each as
     // one range per workgroup size, keeping the largest-first order
     std::vector<RTile> fused;
+    size_t placed = 0, wanted = 0;  // every tile must land in a range
     auto add_ranges = [&](const std::vector<CostTile>& src, bool want_lds, std::vector<TileRange>& dst) {
-        for (int w = 256; w <= 1024; w *= 2) {
+        for (auto& ct : src) wanted += (ct.t.kind == TK_LDS) == want_lds;
+        for (int w = 128; w <= 1024; w *= 2) {
             TileRange r{(int)fused.size(), 0, w, 0, -1, !want_lds};
             for (auto& ct : src) {
                 const BatchClass& k = b->classes[ct.t.cls];
                 if (tile_threads(k, ct.t.kind) != w || (ct.t.kind == TK_LDS) != want_lds)
                     continue;
                 fused.push_back(ct.t);
+                placed++;
                 r.count++;
                 r.lds_bytes = std::max(r.lds_bytes, tile_lds_bytes(k, ct.t.kind));
             }
@@ -381,6 +390,8 @@ static int commit(fftset_batch* b, hipStream_t st)
     add_ranges(ph[0], true, b->lds_r);
     add_ranges(ph1, true, b->lds_r1);
     for (int q = 0; q < 3; ++q) add_ranges(ph[q], false, b->fs_r[q]);
+    if (placed != wanted)
+        return FFTSET_E_UNSUPPORTED;
     for (size_t v = 0; v < job_tiles.size(); ++v) {
         const auto& tl = job_tiles[v];
         TileRange r{(int)per_job.size(), (int)tl.size(), 256, 0};

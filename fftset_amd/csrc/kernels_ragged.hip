@@ -4,6 +4,7 @@
 
 namespace fsa {
 
+hipError_t launch_ragged_128(const RaggedArgs& a, int lds_bytes, hipStream_t st);
 hipError_t launch_ragged_256(const RaggedArgs& a, int lds_bytes, hipStream_t st);
 hipError_t launch_ragged_512(const RaggedArgs& a, int lds_bytes, hipStream_t st);
 hipError_t launch_ragged_1024(const RaggedArgs& a, int lds_bytes, hipStream_t st);
@@ -13,6 +14,7 @@ hipError_t launch_ragged(const RaggedArgs& a, int threads, int lds_bytes, hipStr
     if (a.ntiles <= 0)
         return hipSuccess;
     switch (threads) {
+    case 128: return launch_ragged_128(a, lds_bytes, st);
     case 256: return launch_ragged_256(a, lds_bytes, st);
     case 512: return launch_ragged_512(a, lds_bytes, st);
     case 1024: return launch_ragged_1024(a, lds_bytes, st);

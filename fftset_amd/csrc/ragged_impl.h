@@ -18,9 +18,8 @@ constexpr int kRaggedE = 16;
 constexpr int kIdleLane = 1 << 28;
 
 // FOLD_ST: the FREQOFFSETREAL demodulation folded into the convolution's
-// closing pass too. Only the 256-lane kernel does: folding both sides takes
-// 132 VGPRs, above the 128 that let two 512-lane (or one 1024-lane)
-// workgroups share a CU.
+// closing pass too. Only the 128- and 256-lane kernels do (measured no
+// faster at 512 lanes).
 template <int MOD, int OP, bool FOLD_ST>
 __device__ __forceinline__ void ragged_lds(const RClass& k, const RTile& t, float2* smem)
 {
@@ -195,8 +194,14 @@ __device__ __forceinline__ void ragged_tile(const RaggedArgs& a, int ti, float2*
 // whose workgroups take tiles in table order from a counter, so the
 // largest-first order holds across the whole launch. Tiles of one launch
 // never depend on each other, so no workgroup ever waits on another.
+//
+// At most 128 VGPRs (four waves per SIMD): left to itself the compiler gives
+// the kernel 132 for its FOLD_ST convolution, and with them every tile of
+// the launch, whatever its operation, runs at three waves per SIMD (1440
+// COMPLEX forward tiles 1.5x slower than the uniform kernel at 96). The
+// cap costs no spills.
 template <int THREADS, bool FS>
-__global__ void __launch_bounds__(THREADS) k_ragged(const RaggedArgs a)
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_ragged(const RaggedArgs a)
 {
     extern __shared__ float2 smem[];
     __shared__ int s_tile;

@@ -196,7 +196,7 @@ void fftset_fft_conv_grad_kernel(const struct fftset_fft *first_pass, float *gra
 struct fftset_batch;
 
 #define FFTSET_BATCH_NO_GRAPH      1u  /* launch the kernels directly instead of a captured hipGraph */
-#define FFTSET_BATCH_THREADS_512   2u  /* at least 512-lane workgroups (default 256)                */
+#define FFTSET_BATCH_THREADS_512   2u  /* at least 512-lane workgroups (default: each plan's packing) */
 #define FFTSET_BATCH_SCHED_SERIAL  4u  /* run the tile groups one after another                      */
 #define FFTSET_BATCH_SCHED_PHASED  8u  /* LDS groups beside the first four-step step only            */
 #define FFTSET_BATCH_PERSISTENT   16u  /* resident-size grids taking tiles from a counter, instead of
