@@ -1,6 +1,8 @@
 """Ragged batches on the GPU: mixed sizes, modulations and operations in one
 graph launch, checked transform by transform against the oracle; graph
 replays and the per-size-launch baselines must reproduce the fused launch."""
+import os
+
 import numpy as np
 import pytest
 
@@ -355,3 +357,36 @@ def test_batch_rejects_bad_use(fs):
     with pytest.raises(fftset_amd.FFTSetError):
         b.add(f, fftset_amd.OP_FORWARD, x, x)
     b.destroy()
+
+
+def test_ragged_batch_takes_the_tuned_packing(oracle):
+    # under the MI355X tune table every LDS class is packed as its plan is:
+    # 32 transforms of 48 bins per 128 lanes, 42 of 120 bins per 512 lanes,
+    # one of 1440 bins per 128 lanes, ...; the batch's largest tile is then
+    # 42 x 120 bins (LDS rows of 120 + 120 / 16 + 1 complex values). Each
+    # output equals the drop-in call on the same plan to rounding, and the
+    # oracle.
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    tune = os.path.join(os.path.dirname(fftset_amd.__file__), "tune_mi355x.txt")
+    s = fftset_amd.FFTSet(tune_file=tune)
+    try:
+        fwd, conv, inv = fftset_amd.OP_FORWARD, fftset_amd.OP_CONV, fftset_amd.OP_INVERSE
+        spec = [(CX, 48, fwd, 40), (FO, 48, conv, 40), (CX, 120, fwd, 50), (FO, 120, conv, 45),
+                (CX, 360, fwd, 12), (FO, 360, conv, 11), (CX, 768, fwd, 5), (FO, 768, conv, 6),
+                (CX, 1440, fwd, 3), (FO, 1440, conv, 3), (CX, 1440, inv, 2)]
+        jobs = _make_jobs(np.random.default_rng(23), spec)
+        b = _build(s, jobs)
+        assert b.info()["lds_bytes"] == 42 * (120 + 120 // 16 + 1) * 8
+        b.launch()
+        torch.cuda.synchronize()
+        for jb in jobs:
+            y = jb["dy"].cpu().numpy()
+            drop_in = torch.full_like(jb["dx"], float("nan"))
+            jb["f"].execute(jb["op"], 1, drop_in, jb["dx"], kern=jb["dk"])
+            torch.cuda.synchronize()
+            assert fp64.rel_rms(y, drop_in.cpu().numpy()) <= 2e-6, (jb["mod"], jb["L"], jb["op"])
+            assert fp64.rel_rms(y, _oracle_ref(oracle, jb)) <= REF_TOL, (jb["mod"], jb["L"], jb["op"])
+        b.destroy()
+    finally:
+        s.destroy()
